@@ -1,0 +1,168 @@
+/*
+ * nlm.h -- C-ABI of the MI355X split-step propagation engine behind the
+ * `nonlinearmedium` drop-in (libnlm_hip.so).
+ *
+ * The reference has no C interface: its hot path is the C++ template
+ * _NonlinearMedium::signalSimulationTemplate (src/_NonlinearMedium.hpp:160-239)
+ * driven by runPumpSimulation / runSignalSimulation / computeGreensFunction /
+ * batchSignalSimulation (src/_NonlinearMedium.cpp:225-444, solvers/Chi3.cpp:25-54)
+ * and reached from Python through pybind11 (src/nlmModulePy.cpp:61-98).  Each
+ * entry point below replaces one of those functions; the citation is given
+ * beside it.  The host classes (the pybind11 module `nonlinearmedium`) compute
+ * the per-medium tables exactly as the reference's constructors do and hand
+ * them to nlm_plan_create.
+ *
+ * Conventions
+ *  - complex values are interleaved (re, im) doubles, arrays row-major, the
+ *    memory layout of the reference's Array2Dcd (src/_NonlinearMedium.hpp:13-15);
+ *  - every pointer argument is a HOST pointer owned by the caller, except where
+ *    a name ends in _dev; output pointers may be NULL, in which case the result
+ *    stays resident on the device (readable later with the *_download calls);
+ *  - every function returns NLM_OK or an error kind; nlm_last_error() returns
+ *    the message for the calling thread.  NLM_INVALID_ARGUMENT corresponds to
+ *    the reference's std::invalid_argument, NLM_RUNTIME_ERROR to
+ *    std::runtime_error;
+ *  - transforms follow src/CustomEigenFFT.h:54-137: forward unscaled,
+ *    inverse scaled by 1/n.  The device path supports n = 2^k, 8 <= n <= 8192.
+ *  - a plan is not safe for concurrent calls; distinct plans may be used from
+ *    distinct threads.
+ */
+#ifndef NLM_H
+#define NLM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  NLM_OK = 0,
+  NLM_INVALID_ARGUMENT = 1,
+  NLM_RUNTIME_ERROR = 2,
+  NLM_DEVICE_ERROR = 3,
+};
+
+/* Solver identifiers, in the order of solvers/RegisteredSolvers.hpp:1-17. */
+enum {
+  NLM_CHI3 = 0, NLM_CHI2PDC, NLM_CHI2SFG, NLM_CHI2AFC, NLM_CHI2SFGPDC, NLM_CHI2SFGII,
+  NLM_CHI2PDCII, NLM_CHI2SFGOPA, NLM_CHI2SFGXPM,
+  NLM_CHI2SHG, NLM_CHI2SHGOPA, NLM_CHI2DSFG, NLM_CHI2SHGXPM, NLM_CHI2ASHG, NLM_CHI3GNLSE,
+  NLM_NUM_MODELS
+};
+
+/* Number of complex per-position coefficients each model's nonlinear step
+ * reads from step_coef (index [i][pos][c], pos = 0: z=(i-1)dz, 1: (i-1/2)dz,
+ * 2: i*dz), and of constants in const_coef.  See nlm_plan_desc. */
+int nlm_model_ncoef(int model, int32_t* n_step_coef, int32_t* n_const_coef, int32_t* n_signal, int32_t* n_pump);
+
+typedef struct {
+  int32_t model;
+  int32_t n_freqs;          /* N                                                   */
+  int32_t n_zsteps;         /* nZ (signal rows); pump grids have 2nZ-1 rows        */
+  /* [nS][N] complex: exp(i dz D_s), exp(i dz/2 D_s), exp(-i dz/2 D_s)
+   * (src/_NonlinearMedium.cpp:174-176, src/_NonlinearMedium.hpp:174,187,236) */
+  const double* disp_step;
+  const double* half_in;
+  const double* half_out;
+  /* [nZ][3][n_step_coef] complex: the per-step scalar factors of the model's
+   * DiffEq (poling, nlStep, phase mismatch, beam profile), evaluated on the
+   * host in the reference's operation order.  Row 0 is unused. */
+  const double* step_coef;
+  const double* const_coef; /* [n_const_coef] complex                              */
+  /* Chi3GNLSE only (solvers/Chi3GNLSE.cpp:36,72): [N] complex Raman spectrum and
+   * [N] complex derivative filter nlStep[0] + nlStep[1]*omega; else NULL.      */
+  const double* raman;
+  const double* gnlse_filter;
+} nlm_plan_desc;
+
+typedef struct nlm_plan_s nlm_plan;
+
+/* Uploads the tables of one medium.  Replaces the device-independent setup the
+ * reference does in _NonlinearMedium::_NonlinearMedium
+ * (src/_NonlinearMedium.cpp:8-40); the tables themselves are built by the host. */
+int nlm_plan_create(const nlm_plan_desc* desc, nlm_plan** out);
+int nlm_plan_destroy(nlm_plan* plan);
+
+/* Run this plan's work on `hip_stream` (a hipStream_t); NULL restores the
+ * plan's own stream.  Lets a caller order the engine against its own work. */
+int nlm_plan_set_stream(nlm_plan* plan, void* hip_stream);
+void* nlm_plan_stream(nlm_plan* plan);
+
+/* Replace the step coefficient table (same layout as at creation), e.g. after
+ * a poling change; the FFT and dispersion tables are kept. */
+int nlm_plan_set_coef(nlm_plan* plan, const double* step_coef);
+
+/* Linear pump propagation, _NonlinearMedium::runPumpSimulation
+ * (src/_NonlinearMedium.cpp:225-250).  envelope: [nP][N] time domain;
+ * disp_pump: [nP][N] real D_p(omega); strength: [2nZ-1] beam-profile factor or
+ * NULL for a constant profile. */
+int nlm_pump_linear(nlm_plan* plan, int n_pump, const double* envelope, const double* disp_pump,
+                    double dzp, const double* strength);
+
+/* Nonlinear (self-phase) pump propagation, Chi3::runPumpSimulation
+ * (solvers/Chi3.cpp:25-54).  disp_step_pump: [N] complex exp(i dzp D_p);
+ * rel_intensity: [2nZ-1] complex (dzp/dz * nlStep * profile). */
+int nlm_pump_chi3(nlm_plan* plan, const double* envelope, const double* disp_pump, double dzp,
+                  const double* disp_step_pump, const double* rel_intensity);
+
+/* Pump grid transfer ([2nZ-1][N] complex, time and frequency domain); used by
+ * _NonlinearMedium::setPump(other, ...) (src/_NonlinearMedium.cpp:506-534),
+ * which builds the grid on the host, and to read the grids back. */
+int nlm_pump_upload(nlm_plan* plan, int pump_index, const double* pump_time, const double* pump_freq);
+int nlm_pump_download(nlm_plan* plan, int pump_index, double* pump_time, double* pump_freq);
+
+/* Full z-resolved propagation, runSignalSimulation -> signalSimulationTemplate
+ * with optimized=false (src/_NonlinearMedium.cpp:253-260,
+ * src/_NonlinearMedium.hpp:219-238).  input: in_len complex (N or k*N). */
+int nlm_propagate_grid(nlm_plan* plan, const double* input, int in_len, int in_time_domain, int input_mode);
+/* [nZ][N] complex signal grids of one mode from the last nlm_propagate_grid. */
+int nlm_grid_download(nlm_plan* plan, int mode, double* signal_freq, double* signal_time);
+
+/* Independent propagations, batchSignalSimulation
+ * (src/_NonlinearMedium.cpp:378-444): rows x cols inputs, output rows x
+ * (n_out*N) final rows of the selected modes. */
+int nlm_batch(nlm_plan* plan, const double* inputs, int rows, int cols, int in_time_domain, int input_mode,
+              int n_out, const int32_t* outputs, double* out);
+
+/* Green's functions, computeGreensFunction (src/_NonlinearMedium.cpp:263-375):
+ * 2*N*n_in impulse propagations in one launch, then the transpose / fftshift2 /
+ * normalisation assembly.  block_scale: [min(n_in,n_out)][n_out] complex
+ * factor per shifted block (all ones without normalisation) or NULL.
+ * C, S: (n_out*N) x (n_in*N) complex, or NULL to keep them on the device. */
+int nlm_greens(nlm_plan* plan, int in_time_domain, int n_in, const int32_t* inputs, int n_out,
+               const int32_t* outputs, const double* block_scale, double* C, double* S);
+int nlm_greens_download(nlm_plan* plan, double* C, double* S);
+
+/* Device-resident variants for callers that keep data in HBM (e.g. PyTorch
+ * tensors): same semantics as nlm_batch with device pointers. */
+int nlm_batch_dev(nlm_plan* plan, const double* inputs_dev, int rows, int cols, int in_time_domain,
+                  int input_mode, int n_out, const int32_t* outputs, double* out_dev);
+
+/* Per-launch device timing of the last call (ms), recorded with HIP events on
+ * the plan's stream when enabled: [0] pump, [1] propagation, [2] assembly. */
+int nlm_plan_enable_timing(nlm_plan* plan, int enable);
+int nlm_plan_last_timing(nlm_plan* plan, double ms[3]);
+
+/* Green's function composition of two media, Cascade::computeGreensFunction
+ * (src/Cascade.cpp:128-129): C = C2 C1 + S2 conj(S1), S = C2 S1 + S2 conj(C1).
+ * C2, S2: M x K; C1, S1: K x Ncol; C, S: M x Ncol (host, row-major). */
+int nlm_greens_compose(int M, int K, int Ncol, const double* C2, const double* S2, const double* C1,
+                       const double* S1, double* C, double* S);
+
+/* Standalone transform with the reference's conventions (src/CustomEigenFFT.h):
+ * count rows of length n, forward unscaled / inverse scaled by 1/n. */
+int nlm_fft(int n, int count, const double* in, double* out, int inverse);
+
+/* Synchronise the plan's stream. */
+int nlm_plan_sync(nlm_plan* plan);
+
+const char* nlm_last_error(void);
+
+/* Library version string and device name of the current device. */
+const char* nlm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,76 @@
+"""ctypes binding of the C-ABI in include/nlm.h (libnlm_hip.so).
+
+This is the binding a Python caller without pybind11 would use; the test suite
+uses it to check the exported symbols and to drive the engine directly.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnlm_hip.so")
+
+# every entry point declared in include/nlm.h
+SYMBOLS = [
+    "nlm_model_ncoef", "nlm_plan_create", "nlm_plan_destroy", "nlm_plan_set_stream", "nlm_plan_stream",
+    "nlm_plan_set_coef", "nlm_pump_linear", "nlm_pump_chi3", "nlm_pump_upload", "nlm_pump_download",
+    "nlm_propagate_grid", "nlm_grid_download", "nlm_batch", "nlm_greens", "nlm_greens_download",
+    "nlm_batch_dev", "nlm_plan_enable_timing", "nlm_plan_last_timing", "nlm_greens_compose", "nlm_fft",
+    "nlm_plan_sync", "nlm_last_error", "nlm_version",
+]
+
+_lib = None
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [("model", C.c_int32), ("n_freqs", C.c_int32), ("n_zsteps", C.c_int32),
+                ("disp_step", C.c_void_p), ("half_in", C.c_void_p), ("half_out", C.c_void_p),
+                ("step_coef", C.c_void_p), ("const_coef", C.c_void_p), ("raman", C.c_void_p),
+                ("gnlse_filter", C.c_void_p)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError("libnlm_hip.so is not built; run python -m multimodenonlinearoptics_amd.build")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.nlm_last_error.restype = C.c_char_p
+        L.nlm_version.restype = C.c_char_p
+        L.nlm_fft.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int]
+        L.nlm_model_ncoef.argtypes = [C.c_int, ip, ip, ip, ip]
+        L.nlm_plan_create.argtypes = [C.POINTER(PlanDesc), C.POINTER(vp)]
+        L.nlm_plan_destroy.argtypes = [vp]
+        L.nlm_plan_stream.argtypes = [vp]
+        L.nlm_plan_stream.restype = vp
+        L.nlm_plan_set_stream.argtypes = [vp, vp]
+        L.nlm_plan_enable_timing.argtypes = [vp, C.c_int]
+        L.nlm_plan_last_timing.argtypes = [vp, dp]
+        L.nlm_plan_sync.argtypes = [vp]
+        L.nlm_pump_linear.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp]
+        L.nlm_greens.argtypes = [vp, C.c_int, C.c_int, ip, C.c_int, ip, vp, vp, vp]
+        L.nlm_greens_download.argtypes = [vp, vp, vp]
+        L.nlm_batch.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, vp]
+        L.nlm_batch_dev.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, vp]
+        L.nlm_greens_compose.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc == 0:
+        return
+    msg = lib().nlm_last_error().decode()
+    if rc == 1:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def fft(x, inverse=False):
+    """Device transform of the rows of x (complex128), reference conventions."""
+    import numpy as np
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.complex128))
+    n = x.shape[-1]
+    out = np.empty_like(x)
+    check(lib().nlm_fft(n, x.size // n, x.ctypes.data, out.ctypes.data, int(inverse)))
+    return out

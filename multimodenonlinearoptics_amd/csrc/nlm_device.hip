@@ -1,0 +1,890 @@
+// nlm_device.hip -- MI355X (gfx950) split-step propagation engine and its C-ABI.
+//
+// One workgroup (or one lane group of T = N/8 lanes when N < 512) owns one
+// field for its whole propagation: the field stays in registers in the
+// stride layout of nlm_fft.hpp, the RK4 nonlinear step runs pointwise on it,
+// and the dispersion step is forward FFT -> phase multiply -> inverse FFT with
+// LDS exchanges only between FFT stages.  The z loop is persistent inside the
+// kernel, so a Green's function (2N independent impulse propagations) or a
+// batch is a single launch.
+//
+// Reference path replaced (file:line in mcmahon-lab/MultimodeNonlinearOptics):
+//   signalSimulationTemplate    src/_NonlinearMedium.hpp:160-239  -> k_prop, k_prop_gnlse
+//   runPumpSimulation           src/_NonlinearMedium.cpp:225-250  -> k_pump_linear
+//   Chi3::runPumpSimulation     solvers/Chi3.cpp:25-54            -> k_pump_chi3
+//   computeGreensFunction       src/_NonlinearMedium.cpp:263-375  -> nlm_greens (k_prop + k_greens_assemble)
+//   batchSignalSimulation       src/_NonlinearMedium.cpp:378-444  -> nlm_batch
+//   runSignalSimulation         src/_NonlinearMedium.cpp:253-260  -> nlm_propagate_grid
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nlm.h"
+#include "nlm_prop.hpp"
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+static thread_local std::string g_err;
+
+static int set_err(int kind, const std::string& msg) {
+  g_err = msg;
+  return kind;
+}
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return set_err(NLM_DEVICE_ERROR, std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" const char* nlm_last_error(void) { return g_err.c_str(); }
+extern "C" const char* nlm_version(void) { return "nlm-hip 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------------------
+// pump kernels
+// ---------------------------------------------------------------------------
+
+// Rows of the linear pump grid (src/_NonlinearMedium.cpp:225-250):
+// row i: freq = F0 * exp(i (i*dzp) D_p), time = IFFT(freq); both * strength[i].
+template <int L>
+__global__ void __launch_bounds__(Launch<L>::THREADS)
+    k_pump_linear(const cplx* __restrict__ F0, const cplx* __restrict__ env, const double* __restrict__ Dp,
+                  double dzp, const double* __restrict__ strength, int nZP, cplx* __restrict__ pumpF,
+                  cplx* __restrict__ pumpT, const cplx* __restrict__ twt) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T, FPB = Launch<L>::FPB;
+  __shared__ cplx lds[FPB * F::lds_elems];
+  const int lf = threadIdx.x / T, t = threadIdx.x % T;
+  const int row = blockIdx.x * FPB + lf;
+  const bool active = row < nZP;
+  cplx* buf = lds + lf * F::lds_elems;
+  FftTwiddles<L> tw;
+  fft_load_twiddles<L>(tw, twt, t);
+  const int i = active ? row : 0;
+  cplx f[8], tm[8];
+  const double zi = (double)i * dzp;
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    const int k = t + T * m;
+    if (i == 0) {
+      f[m] = F0[k];
+    } else {
+      double s, c;
+      sincos(zi * Dp[k], &s, &c);
+      f[m] = F0[k] * mk(c, s);
+    }
+    tm[m] = f[m];
+  }
+  fft_field<L, true>(tm, buf, t, tw);
+  scale_inv<L>(tm);
+  if (i == 0) {
+#pragma unroll
+    for (int m = 0; m < 8; m++) tm[m] = env[t + T * m];
+  }
+  if (strength) {
+    const double s = strength[i];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      f[m] = f[m] * s;
+      tm[m] = tm[m] * s;
+    }
+  }
+  store_row<L>(pumpF + (size_t)i * N, f, t, active);
+  store_row<L>(pumpT + (size_t)i * N, tm, t, active);
+}
+
+// Chi3 nonlinear pump, serial over 2nZ-1 rows (solvers/Chi3.cpp:25-54).
+template <int L>
+__global__ void __launch_bounds__(FftShape<L>::T) k_pump_chi3(const cplx* __restrict__ env, const double* __restrict__ Dp,
+                                                              double dzp, const cplx* __restrict__ dstepP,
+                                                              const cplx* __restrict__ relI, int nZP,
+                                                              cplx* __restrict__ pumpF, cplx* __restrict__ pumpT,
+                                                              const cplx* __restrict__ twt) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T;
+  __shared__ cplx buf[F::lds_elems];
+  const int t = threadIdx.x;
+  FftTwiddles<L> tw;
+  fft_load_twiddles<L>(tw, twt, t);
+  cplx v[8], dsp[8];
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    v[m] = env[t + T * m];
+    dsp[m] = dstepP[t + T * m];
+  }
+  fft_field<L, false>(v, buf, t, tw);
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    double s, c;
+    sincos((0.5 * dzp) * Dp[t + T * m], &s, &c);
+    v[m] = v[m] * mk(c, s);
+  }
+  store_row<L>(pumpF, v, t, true);
+  fft_field<L, true>(v, buf, t, tw);
+  scale_inv<L>(v);
+  store_row<L>(pumpT, v, t, true);
+  for (int i = 1; i < nZP; i++) {
+    const cplx r = relI[i - 1];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      const double a2 = abs2(v[m]);
+      const double e = exp(r.re * a2);
+      double s, c;
+      sincos(r.im * a2, &s, &c);
+      v[m] = v[m] * mk(e * c, e * s);
+    }
+    fft_field<L, false>(v, buf, t, tw);
+#pragma unroll
+    for (int m = 0; m < 8; m++) v[m] = v[m] * dsp[m];
+    if (i == nZP - 1) {
+#pragma unroll
+      for (int m = 0; m < 8; m++) {
+        double s, c;
+        sincos((-0.5 * dzp) * Dp[t + T * m], &s, &c);
+        v[m] = v[m] * mk(c, s);
+      }
+    }
+    store_row<L>(pumpF + (size_t)i * N, v, t, true);
+    fft_field<L, true>(v, buf, t, tw);
+    scale_inv<L>(v);
+    store_row<L>(pumpT + (size_t)i * N, v, t, true);
+  }
+}
+
+// Rows of a standalone transform.
+template <int L, bool INV>
+__global__ void __launch_bounds__(Launch<L>::THREADS)
+    k_fft_rows(const cplx* __restrict__ in, cplx* __restrict__ out, int count, const cplx* __restrict__ twt) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T, FPB = Launch<L>::FPB;
+  __shared__ cplx lds[FPB * F::lds_elems];
+  const int lf = threadIdx.x / T, t = threadIdx.x % T;
+  const int row = blockIdx.x * FPB + lf;
+  const bool active = row < count;
+  FftTwiddles<L> tw;
+  fft_load_twiddles<L>(tw, twt, t);
+  cplx v[8];
+#pragma unroll
+  for (int m = 0; m < 8; m++) v[m] = active ? in[(size_t)row * N + t + T * m] : mk(0, 0);
+  fft_field<L, INV>(v, lds + lf * F::lds_elems, t, tw);
+  if (INV) scale_inv<L>(v);
+  store_row<L>(out + (size_t)row * N, v, t, active);
+}
+
+// Green's matrix assembly (src/_NonlinearMedium.cpp:333-372):
+// C = 1/2 G(e) - i/2 G(i e), S = 1/2 G(e) + i/2 G(i e), transposed, each block
+// fftshift2-ed (blocks im < nb), off-diagonal blocks optionally scaled.
+__global__ void k_greens_assemble(const cplx* __restrict__ G, cplx* __restrict__ C, cplx* __restrict__ S, int N,
+                                  int nIn, int nOut, int nb, const cplx* __restrict__ scale) {
+  const long rows = (long)nOut * N, cols = (long)nIn * N;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * cols) return;
+  const long R = idx / cols, Cc = idx % cols;
+  const int om = (int)(R / N), r = (int)(R % N), im = (int)(Cc / N), c = (int)(Cc % N);
+  const int h = N / 2;
+  long si, sc;
+  bool doScale = false;
+  if (im < nb) {
+    si = (long)im * N + (c + h) % N;
+    sc = (long)om * N + (r + h) % N;
+    doScale = scale != nullptr && im != om;
+  } else {
+    si = (long)im * N + c;
+    sc = (long)om * N + r;
+  }
+  const long W = (long)nOut * N;
+  const cplx g0 = G[(2 * si) * W + sc], g1 = G[(2 * si + 1) * W + sc];
+  cplx cv = mk(0.5 * g0.re + 0.5 * g1.im, 0.5 * g0.im - 0.5 * g1.re);
+  cplx sv = mk(0.5 * g0.re - 0.5 * g1.im, 0.5 * g0.im + 0.5 * g1.re);
+  if (doScale) {
+    const cplx s = scale[im * nOut + om];
+    cv = cv * s;
+    sv = sv * s;
+  }
+  C[idx] = cv;
+  S[idx] = sv;
+}
+
+// ---------------------------------------------------------------------------
+// dispatch over (model, log2 N)
+// ---------------------------------------------------------------------------
+
+static constexpr int LMIN = 3, LMAX = 13;
+
+// One launcher table per model, each built in its own translation unit.
+prop_fn nlm_pick_prop_0(int), nlm_pick_prop_1(int), nlm_pick_prop_2(int), nlm_pick_prop_4(int),
+    nlm_pick_prop_5(int), nlm_pick_prop_6(int), nlm_pick_prop_7(int), nlm_pick_prop_8(int),
+    nlm_pick_prop_9(int), nlm_pick_prop_10(int), nlm_pick_prop_11(int), nlm_pick_prop_12(int),
+    nlm_pick_prop_14(int);
+
+// Chi2AFC shares Chi2SFG's kernel and Chi2ASHG shares Chi2SHG's: their
+// differences (no poling, chirped mismatch) live in the host coefficient table.
+prop_fn nlm_pick_prop(int model, int L) {
+  switch (model) {
+    case NLM_CHI3: return nlm_pick_prop_0(L);
+    case NLM_CHI2PDC: return nlm_pick_prop_1(L);
+    case NLM_CHI2SFG: return nlm_pick_prop_2(L);
+    case NLM_CHI2AFC: return nlm_pick_prop_2(L);
+    case NLM_CHI2SFGPDC: return nlm_pick_prop_4(L);
+    case NLM_CHI2SFGII: return nlm_pick_prop_5(L);
+    case NLM_CHI2PDCII: return nlm_pick_prop_6(L);
+    case NLM_CHI2SFGOPA: return nlm_pick_prop_7(L);
+    case NLM_CHI2SFGXPM: return nlm_pick_prop_8(L);
+    case NLM_CHI2SHG: return nlm_pick_prop_9(L);
+    case NLM_CHI2SHGOPA: return nlm_pick_prop_10(L);
+    case NLM_CHI2DSFG: return nlm_pick_prop_11(L);
+    case NLM_CHI2SHGXPM: return nlm_pick_prop_12(L);
+    case NLM_CHI2ASHG: return nlm_pick_prop_9(L);
+    case NLM_CHI3GNLSE: return nlm_pick_prop_14(L);
+  }
+  return nullptr;
+}
+
+typedef void (*pump_lin_fn)(const cplx*, const cplx*, const double*, double, const double*, int, cplx*, cplx*,
+                            const cplx*, hipStream_t);
+typedef void (*pump_chi3_fn)(const cplx*, const double*, double, const cplx*, const cplx*, int, cplx*, cplx*,
+                             const cplx*, hipStream_t);
+typedef void (*fft_fn)(const cplx*, cplx*, int, const cplx*, bool, hipStream_t);
+
+template <int L>
+static void launch_pump_linear(const cplx* F0, const cplx* env, const double* Dp, double dzp, const double* st,
+                               int nZP, cplx* pF, cplx* pT, const cplx* tw, hipStream_t s) {
+  const int grid = (nZP + Launch<L>::FPB - 1) / Launch<L>::FPB;
+  hipLaunchKernelGGL((k_pump_linear<L>), dim3(grid), dim3(Launch<L>::THREADS), 0, s, F0, env, Dp, dzp, st, nZP, pF,
+                     pT, tw);
+}
+template <int L>
+static void launch_pump_chi3(const cplx* env, const double* Dp, double dzp, const cplx* dsp, const cplx* relI,
+                             int nZP, cplx* pF, cplx* pT, const cplx* tw, hipStream_t s) {
+  hipLaunchKernelGGL((k_pump_chi3<L>), dim3(1), dim3(FftShape<L>::T), 0, s, env, Dp, dzp, dsp, relI, nZP, pF, pT, tw);
+}
+template <int L>
+static void launch_fft(const cplx* in, cplx* out, int count, const cplx* tw, bool inv, hipStream_t s) {
+  const int grid = (count + Launch<L>::FPB - 1) / Launch<L>::FPB;
+  if (inv) hipLaunchKernelGGL((k_fft_rows<L, true>), dim3(grid), dim3(Launch<L>::THREADS), 0, s, in, out, count, tw);
+  else hipLaunchKernelGGL((k_fft_rows<L, false>), dim3(grid), dim3(Launch<L>::THREADS), 0, s, in, out, count, tw);
+}
+
+#define NLM_L_SWITCH(FN, L)          \
+  switch (L) {                        \
+    case 3: return &FN<3>;            \
+    case 4: return &FN<4>;            \
+    case 5: return &FN<5>;            \
+    case 6: return &FN<6>;            \
+    case 7: return &FN<7>;            \
+    case 8: return &FN<8>;            \
+    case 9: return &FN<9>;            \
+    case 10: return &FN<10>;          \
+    case 11: return &FN<11>;          \
+    case 12: return &FN<12>;          \
+    case 13: return &FN<13>;          \
+  }                                   \
+  return nullptr;
+
+static pump_lin_fn pick_pump_linear(int L) { NLM_L_SWITCH(launch_pump_linear, L) }
+static pump_chi3_fn pick_pump_chi3(int L) { NLM_L_SWITCH(launch_pump_chi3, L) }
+static fft_fn pick_fft(int L) { NLM_L_SWITCH(launch_fft, L) }
+
+// ---------------------------------------------------------------------------
+// host-side helpers
+// ---------------------------------------------------------------------------
+
+static int log2_exact(int n) {
+  if (n < 8 || (n & (n - 1))) return -1;
+  int l = 0;
+  while ((1 << l) < n) l++;
+  return l;
+}
+
+static int check_n(int n, int* L) {
+  *L = log2_exact(n);
+  if (*L < LMIN || *L > LMAX)
+    return set_err(NLM_INVALID_ARGUMENT, "nFreqs must be a power of two between 8 and 8192 on the MI355X path (got " +
+                                             std::to_string(n) + ")");
+  return NLM_OK;
+}
+
+// W_N^k = exp(-2 pi i k/N), evaluated in long double and rounded once.
+static std::vector<cplx> twiddle_table(int n) {
+  std::vector<cplx> w(n);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int k = 0; k < n; k++) {
+    long double a = -2.0L * pi * (long double)k / (long double)n;
+    w[k] = mk((double)cosl(a), (double)sinl(a));
+  }
+  return w;
+}
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int ensure(size_t count) {
+    if (count <= n && p) return NLM_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) return NLM_OK;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) return set_err(NLM_DEVICE_ERROR, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    n = count;
+    return NLM_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Twiddle tables shared by all plans of one device and size.
+static std::mutex g_tw_mu;
+static std::map<std::pair<int, int>, cplx*> g_tw;
+
+static int get_twiddles(int n, const cplx** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto key = std::make_pair(dev, n);
+  auto it = g_tw.find(key);
+  if (it != g_tw.end()) {
+    *out = it->second;
+    return NLM_OK;
+  }
+  std::vector<cplx> w = twiddle_table(n);
+  cplx* d = nullptr;
+  HIP_TRY(hipMalloc((void**)&d, n * sizeof(cplx)));
+  HIP_TRY(hipMemcpy(d, w.data(), n * sizeof(cplx), hipMemcpyHostToDevice));
+  g_tw[key] = d;
+  *out = d;
+  return NLM_OK;
+}
+
+struct nlm_plan_s {
+  int model = 0, N = 0, L = 0, nZ = 0, nS = 0, nP = 0, NC = 0, NCC = 0;
+  int device = 0;
+  hipStream_t own = nullptr, stream = nullptr;
+  const cplx* tw = nullptr;
+  DevBuf<cplx> dstep, halfIn, halfOut, coef, cconst, raman, filt;
+  DevBuf<cplx> pumpT[2], pumpF[2];
+  DevBuf<cplx> gridF, gridT;
+  DevBuf<cplx> G, C, S, in, out, env, F0, dsp, relI;
+  DevBuf<double> Dp, strength;
+  int greensRows = 0, greensCols = 0;
+  bool timing = false;
+  hipEvent_t ev[6] = {};
+  double last_ms[3] = {0, 0, 0};
+  std::vector<cplx> host_tmp;
+};
+
+static const int kModelNS[NLM_NUM_MODELS] = {1, 1, 2, 2, 2, 4, 2, 2, 2, 2, 4, 3, 2, 2, 1};
+static const int kModelNP[NLM_NUM_MODELS] = {1, 1, 1, 1, 1, 1, 1, 2, 1, 0, 0, 0, 0, 0, 0};
+static const int kModelNC[NLM_NUM_MODELS] = {0, 1, 2, 2, 4, 7, 2, 4, 2, 2, 6, 3, 4, 2, 1};
+static const int kModelNCC[NLM_NUM_MODELS] = {1, 0, 0, 0, 0, 0, 0, 0, 2, 0, 0, 0, 0, 0, 0};
+
+extern "C" int nlm_model_ncoef(int model, int32_t* nc, int32_t* ncc, int32_t* ns, int32_t* np) {
+  if (model < 0 || model >= NLM_NUM_MODELS) return set_err(NLM_INVALID_ARGUMENT, "unknown model");
+  if (nc) *nc = kModelNC[model];
+  if (ncc) *ncc = kModelNCC[model];
+  if (ns) *ns = kModelNS[model];
+  if (np) *np = kModelNP[model];
+  return NLM_OK;
+}
+
+static int upload(DevBuf<cplx>& b, const double* src, size_t count, hipStream_t s) {
+  int e = b.ensure(count);
+  if (e) return e;
+  if (count && src) HIP_TRY(hipMemcpyAsync(b.p, src, count * sizeof(cplx), hipMemcpyHostToDevice, s));
+  return NLM_OK;
+}
+
+static void tic(nlm_plan* p, int slot) {
+  if (p->timing) (void)hipEventRecord(p->ev[slot], p->stream);
+}
+
+extern "C" int nlm_plan_create(const nlm_plan_desc* d, nlm_plan** out) {
+  *out = nullptr;
+  if (!d) return set_err(NLM_INVALID_ARGUMENT, "null plan descriptor");
+  if (d->model < 0 || d->model >= NLM_NUM_MODELS) return set_err(NLM_INVALID_ARGUMENT, "unknown model");
+  int L;
+  if (int e = check_n(d->n_freqs, &L)) return e;
+  if (d->n_zsteps < 1) return set_err(NLM_INVALID_ARGUMENT, "Zero steps");
+  if (d->model == NLM_CHI3GNLSE && (!d->raman || !d->gnlse_filter))
+    return set_err(NLM_INVALID_ARGUMENT, "Chi3GNLSE plan needs the Raman spectrum and derivative filter");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return set_err(NLM_DEVICE_ERROR, "no HIP device available: the MI355X engine needs a GPU");
+  nlm_plan* p = new nlm_plan();
+  p->model = d->model;
+  p->N = d->n_freqs;
+  p->L = L;
+  p->nZ = d->n_zsteps;
+  p->nS = kModelNS[p->model];
+  p->nP = kModelNP[p->model];
+  p->NC = kModelNC[p->model];
+  p->NCC = kModelNCC[p->model];
+  int e = NLM_OK;
+  auto fail = [&](int code) {
+    nlm_plan_destroy(p);
+    return code;
+  };
+  if (hipGetDevice(&p->device) != hipSuccess) return fail(set_err(NLM_DEVICE_ERROR, "hipGetDevice failed"));
+  if (hipStreamCreateWithFlags(&p->own, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_err(NLM_DEVICE_ERROR, "hipStreamCreate failed"));
+  p->stream = p->own;
+  for (auto& ev : p->ev)
+    if (hipEventCreate(&ev) != hipSuccess) return fail(set_err(NLM_DEVICE_ERROR, "hipEventCreate failed"));
+  if ((e = get_twiddles(p->N, &p->tw))) return fail(e);
+  const size_t nsN = (size_t)p->nS * p->N;
+  if ((e = upload(p->dstep, d->disp_step, nsN, p->stream))) return fail(e);
+  if ((e = upload(p->halfIn, d->half_in, nsN, p->stream))) return fail(e);
+  if ((e = upload(p->halfOut, d->half_out, nsN, p->stream))) return fail(e);
+  if ((e = upload(p->coef, d->step_coef, (size_t)p->nZ * 3 * (p->NC > 0 ? p->NC : 1), p->stream))) return fail(e);
+  if ((e = upload(p->cconst, d->const_coef, p->NCC > 0 ? p->NCC : 1, p->stream))) return fail(e);
+  if (p->model == NLM_CHI3GNLSE) {
+    if ((e = upload(p->raman, d->raman, p->N, p->stream))) return fail(e);
+    if ((e = upload(p->filt, d->gnlse_filter, p->N, p->stream))) return fail(e);
+  }
+  if (hipStreamSynchronize(p->stream) != hipSuccess) return fail(set_err(NLM_DEVICE_ERROR, "upload failed"));
+  *out = p;
+  return NLM_OK;
+}
+
+extern "C" int nlm_plan_destroy(nlm_plan* p) {
+  if (!p) return NLM_OK;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  DevBuf<cplx>* bufs[] = {&p->dstep, &p->halfIn, &p->halfOut, &p->coef, &p->cconst, &p->raman, &p->filt,
+                          &p->pumpT[0], &p->pumpT[1], &p->pumpF[0], &p->pumpF[1], &p->gridF, &p->gridT,
+                          &p->G, &p->C, &p->S, &p->in, &p->out, &p->env, &p->F0, &p->dsp, &p->relI};
+  for (auto* b : bufs) b->release();
+  p->Dp.release();
+  p->strength.release();
+  for (auto& ev : p->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (p->own) (void)hipStreamDestroy(p->own);
+  delete p;
+  return NLM_OK;
+}
+
+extern "C" int nlm_plan_set_stream(nlm_plan* p, void* s) {
+  if (!p) return set_err(NLM_INVALID_ARGUMENT, "null plan");
+  p->stream = s ? (hipStream_t)s : p->own;
+  return NLM_OK;
+}
+
+extern "C" void* nlm_plan_stream(nlm_plan* p) { return p ? (void*)p->stream : nullptr; }
+
+extern "C" int nlm_plan_sync(nlm_plan* p) {
+  if (!p) return set_err(NLM_INVALID_ARGUMENT, "null plan");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_plan_set_coef(nlm_plan* p, const double* coef) {
+  if (!p || !coef) return set_err(NLM_INVALID_ARGUMENT, "null argument");
+  return upload(p->coef, coef, (size_t)p->nZ * 3 * (p->NC > 0 ? p->NC : 1), p->stream);
+}
+
+extern "C" int nlm_plan_enable_timing(nlm_plan* p, int en) {
+  if (!p) return set_err(NLM_INVALID_ARGUMENT, "null plan");
+  p->timing = en != 0;
+  return NLM_OK;
+}
+
+extern "C" int nlm_plan_last_timing(nlm_plan* p, double ms[3]) {
+  if (!p) return set_err(NLM_INVALID_ARGUMENT, "null plan");
+  for (int i = 0; i < 3; i++) ms[i] = p->last_ms[i];
+  return NLM_OK;
+}
+
+static void collect_timing(nlm_plan* p, int a0, int a1, int slot) {
+  if (!p->timing) return;
+  float ms = 0;
+  if (hipEventSynchronize(p->ev[a1]) == hipSuccess && hipEventElapsedTime(&ms, p->ev[a0], p->ev[a1]) == hipSuccess)
+    p->last_ms[slot] = ms;
+}
+
+static int ensure_pumps(nlm_plan* p) {
+  const size_t nZP = 2 * (size_t)p->nZ - 1;
+  for (int q = 0; q < p->nP; q++) {
+    if (int e = p->pumpT[q].ensure(nZP * p->N)) return e;
+    if (int e = p->pumpF[q].ensure(nZP * p->N)) return e;
+  }
+  return NLM_OK;
+}
+
+extern "C" int nlm_pump_linear(nlm_plan* p, int n_pump, const double* envelope, const double* disp_pump, double dzp,
+                               const double* strength) {
+  if (!p || !envelope || !disp_pump) return set_err(NLM_INVALID_ARGUMENT, "null argument");
+  if (n_pump != p->nP || p->nP == 0) return set_err(NLM_INVALID_ARGUMENT, "pump count does not match the model");
+  if (int e = ensure_pumps(p)) return e;
+  const int N = p->N, nZP = 2 * p->nZ - 1;
+  if (int e = upload(p->env, envelope, (size_t)p->nP * N, p->stream)) return e;
+  if (int e = p->Dp.ensure((size_t)p->nP * N)) return e;
+  HIP_TRY(hipMemcpyAsync(p->Dp.p, disp_pump, sizeof(double) * p->nP * N, hipMemcpyHostToDevice, p->stream));
+  if (strength) {
+    if (int e = p->strength.ensure(nZP)) return e;
+    HIP_TRY(hipMemcpyAsync(p->strength.p, strength, sizeof(double) * nZP, hipMemcpyHostToDevice, p->stream));
+  }
+  if (int e = p->F0.ensure((size_t)p->nP * N)) return e;
+  tic(p, 0);
+  pick_fft(p->L)(p->env.p, p->F0.p, p->nP, p->tw, false, p->stream);
+  for (int q = 0; q < p->nP; q++)
+    pick_pump_linear(p->L)(p->F0.p + (size_t)q * N, p->env.p + (size_t)q * N, p->Dp.p + (size_t)q * N, dzp,
+                           strength ? p->strength.p : nullptr, nZP, p->pumpF[q].p, p->pumpT[q].p, p->tw, p->stream);
+  HIP_TRY(hipGetLastError());
+  tic(p, 1);
+  if (p->timing) collect_timing(p, 0, 1, 0);
+  return NLM_OK;
+}
+
+extern "C" int nlm_pump_chi3(nlm_plan* p, const double* envelope, const double* disp_pump, double dzp,
+                             const double* disp_step_pump, const double* rel_intensity) {
+  if (!p || !envelope || !disp_pump || !disp_step_pump || !rel_intensity)
+    return set_err(NLM_INVALID_ARGUMENT, "null argument");
+  if (p->nP != 1) return set_err(NLM_INVALID_ARGUMENT, "model has no single pump");
+  if (int e = ensure_pumps(p)) return e;
+  const int N = p->N, nZP = 2 * p->nZ - 1;
+  if (int e = upload(p->env, envelope, N, p->stream)) return e;
+  if (int e = p->Dp.ensure(N)) return e;
+  HIP_TRY(hipMemcpyAsync(p->Dp.p, disp_pump, sizeof(double) * N, hipMemcpyHostToDevice, p->stream));
+  if (int e = upload(p->dsp, disp_step_pump, N, p->stream)) return e;
+  if (int e = upload(p->relI, rel_intensity, nZP, p->stream)) return e;
+  tic(p, 0);
+  pick_pump_chi3(p->L)(p->env.p, p->Dp.p, dzp, p->dsp.p, p->relI.p, nZP, p->pumpF[0].p, p->pumpT[0].p, p->tw,
+                       p->stream);
+  HIP_TRY(hipGetLastError());
+  tic(p, 1);
+  if (p->timing) collect_timing(p, 0, 1, 0);
+  return NLM_OK;
+}
+
+extern "C" int nlm_pump_upload(nlm_plan* p, int q, const double* pump_time, const double* pump_freq) {
+  if (!p) return set_err(NLM_INVALID_ARGUMENT, "null plan");
+  if (q < 0 || q >= p->nP) return set_err(NLM_INVALID_ARGUMENT, "Invalid pump index");
+  if (int e = ensure_pumps(p)) return e;
+  const size_t cnt = (2 * (size_t)p->nZ - 1) * p->N;
+  if (pump_time) HIP_TRY(hipMemcpyAsync(p->pumpT[q].p, pump_time, cnt * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
+  if (pump_freq) HIP_TRY(hipMemcpyAsync(p->pumpF[q].p, pump_freq, cnt * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_pump_download(nlm_plan* p, int q, double* pump_time, double* pump_freq) {
+  if (!p) return set_err(NLM_INVALID_ARGUMENT, "null plan");
+  if (q < 0 || q >= p->nP) return set_err(NLM_INVALID_ARGUMENT, "Invalid pump index");
+  if (!p->pumpT[q].p) return set_err(NLM_RUNTIME_ERROR, "pump grid has not been computed");
+  const size_t cnt = (2 * (size_t)p->nZ - 1) * p->N;
+  if (pump_time) HIP_TRY(hipMemcpyAsync(pump_time, p->pumpT[q].p, cnt * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  if (pump_freq) HIP_TRY(hipMemcpyAsync(pump_freq, p->pumpF[q].p, cnt * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return NLM_OK;
+}
+
+static void fill_common(nlm_plan* p, KParams& kp) {
+  memset(&kp, 0, sizeof kp);
+  kp.nZ = p->nZ;
+  kp.tw = p->tw;
+  kp.dstep = p->dstep.p;
+  kp.halfIn = p->halfIn.p;
+  kp.halfOut = p->halfOut.p;
+  kp.coef = p->coef.p;
+  kp.cconst = p->cconst.p;
+  kp.pump0 = p->nP > 0 ? p->pumpT[0].p : nullptr;
+  kp.pump1 = p->nP > 1 ? p->pumpT[1].p : nullptr;
+  kp.raman = p->raman.p;
+  kp.filt = p->filt.p;
+  for (int q = 0; q < 4; q++) {
+    kp.inPos[q] = -1;
+    kp.outPos[q] = -1;
+  }
+}
+
+static int check_pumps_ready(nlm_plan* p) {
+  for (int q = 0; q < p->nP; q++)
+    if (!p->pumpT[q].p) return set_err(NLM_RUNTIME_ERROR, "pump grid has not been computed (run the pump simulation first)");
+  return NLM_OK;
+}
+
+// Input channel of each mode (src/_NonlinearMedium.hpp:164-198).
+static void input_channels(int nS, int nCh, int inputMode, int* inPos) {
+  if (nCh > 1) inputMode = 0;
+  if (nS <= 1) inputMode = 0;
+  for (int q = 0; q < nS; q++) {
+    if (q == inputMode) inPos[q] = 0;
+    else if (inputMode < 1 && q < nCh) inPos[q] = q;
+    else inPos[q] = -1;
+  }
+}
+
+static int launch_prop_kernel(nlm_plan* p, const KParams& kp) {
+  prop_fn fn = nlm_pick_prop(p->model, p->L);
+  if (!fn) return set_err(NLM_INVALID_ARGUMENT, "unsupported model / size");
+  int fpb = 1, T = p->N / 8;
+  if (T < 64) fpb = 64 / T;
+  const int grid = (kp.nFields + fpb - 1) / fpb;
+  fn(kp, grid, p->stream);
+  HIP_TRY(hipGetLastError());
+  return NLM_OK;
+}
+
+extern "C" int nlm_propagate_grid(nlm_plan* p, const double* input, int in_len, int in_time, int input_mode) {
+  if (!p || !input) return set_err(NLM_INVALID_ARGUMENT, "null argument");
+  const int N = p->N;
+  if (in_len % N != 0 || in_len / N == 0 || in_len / N > p->nS)
+    return set_err(NLM_INVALID_ARGUMENT, "inputProf array size does not match number of frequency/time bins");
+  if (input_mode < 0 || input_mode >= p->nS)
+    return set_err(NLM_INVALID_ARGUMENT, "inputModes does not match any mode in the system");
+  if (int e = check_pumps_ready(p)) return e;
+  const size_t g = (size_t)p->nS * p->nZ * N;
+  if (int e = p->gridF.ensure(g)) return e;
+  if (int e = p->gridT.ensure(g)) return e;
+  if (int e = upload(p->in, input, in_len, p->stream)) return e;
+  KParams kp;
+  fill_common(p, kp);
+  kp.kind = KIND_GRID;
+  kp.nFields = 1;
+  kp.inTime = in_time;
+  kp.inCols = in_len;
+  kp.in = p->in.p;
+  kp.gridF = p->gridF.p;
+  kp.gridT = p->gridT.p;
+  input_channels(p->nS, in_len / N, input_mode, kp.inPos);
+  tic(p, 2);
+  if (int e = launch_prop_kernel(p, kp)) return e;
+  tic(p, 3);
+  if (p->timing) collect_timing(p, 2, 3, 1);
+  return NLM_OK;
+}
+
+extern "C" int nlm_grid_download(nlm_plan* p, int mode, double* sf, double* st) {
+  if (!p) return set_err(NLM_INVALID_ARGUMENT, "null plan");
+  if (mode < 0 || mode >= p->nS) return set_err(NLM_INVALID_ARGUMENT, "mode index out of range");
+  if (!p->gridF.p) return set_err(NLM_RUNTIME_ERROR, "no signal grid has been computed");
+  const size_t cnt = (size_t)p->nZ * p->N, off = (size_t)mode * cnt;
+  if (sf) HIP_TRY(hipMemcpyAsync(sf, p->gridF.p + off, cnt * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  if (st) HIP_TRY(hipMemcpyAsync(st, p->gridT.p + off, cnt * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return NLM_OK;
+}
+
+static int batch_common(nlm_plan* p, const cplx* in_dev, int rows, int cols, int in_time, int input_mode, int n_out,
+                        const int32_t* outputs, cplx* out_dev) {
+  const int N = p->N;
+  if (cols % N != 0 || cols / N == 0 || cols / N > p->nS) return set_err(NLM_INVALID_ARGUMENT, "Signals not of correct length!");
+  if (input_mode < 0 || input_mode >= p->nS)
+    return set_err(NLM_INVALID_ARGUMENT, "inputModes does not match any mode in the system");
+  if (n_out < 1 || n_out > p->nS) return set_err(NLM_INVALID_ARGUMENT, "Requested no outputs!");
+  if (int e = check_pumps_ready(p)) return e;
+  KParams kp;
+  fill_common(p, kp);
+  kp.kind = KIND_BATCH;
+  kp.nFields = rows;
+  kp.inTime = in_time;
+  kp.outTime = in_time;
+  kp.inCols = cols;
+  kp.in = in_dev;
+  kp.out = out_dev;
+  kp.nOut = n_out;
+  for (int om = 0; om < n_out; om++) {
+    if (outputs[om] < 0 || outputs[om] >= p->nS) return set_err(NLM_INVALID_ARGUMENT, "output mode out of range");
+    kp.outPos[outputs[om]] = om;
+  }
+  input_channels(p->nS, cols / N, input_mode, kp.inPos);
+  if (rows == 0) return NLM_OK;
+  tic(p, 2);
+  if (int e = launch_prop_kernel(p, kp)) return e;
+  tic(p, 3);
+  if (p->timing) collect_timing(p, 2, 3, 1);
+  return NLM_OK;
+}
+
+extern "C" int nlm_batch(nlm_plan* p, const double* inputs, int rows, int cols, int in_time, int input_mode, int n_out,
+                         const int32_t* outputs, double* out) {
+  if (!p || (!inputs && rows > 0) || !outputs) return set_err(NLM_INVALID_ARGUMENT, "null argument");
+  if (rows < 0) return set_err(NLM_INVALID_ARGUMENT, "negative row count");
+  if (int e = upload(p->in, inputs, (size_t)rows * cols, p->stream)) return e;
+  if (int e = p->out.ensure((size_t)rows * n_out * p->N)) return e;
+  if (int e = batch_common(p, p->in.p, rows, cols, in_time, input_mode, n_out, outputs, p->out.p)) return e;
+  if (out && rows > 0)
+    HIP_TRY(hipMemcpyAsync(out, p->out.p, (size_t)rows * n_out * p->N * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_batch_dev(nlm_plan* p, const double* in_dev, int rows, int cols, int in_time, int input_mode,
+                             int n_out, const int32_t* outputs, double* out_dev) {
+  if (!p || !outputs || (rows > 0 && (!in_dev || !out_dev))) return set_err(NLM_INVALID_ARGUMENT, "null argument");
+  return batch_common(p, (const cplx*)in_dev, rows, cols, in_time, input_mode, n_out, outputs, (cplx*)out_dev);
+}
+
+extern "C" int nlm_greens(nlm_plan* p, int in_time, int n_in, const int32_t* inputs, int n_out, const int32_t* outputs,
+                          const double* block_scale, double* C, double* S) {
+  if (!p || !inputs || !outputs) return set_err(NLM_INVALID_ARGUMENT, "null argument");
+  if (p->model >= NLM_CHI2SHG) return set_err(NLM_RUNTIME_ERROR, "Object does not have this method.");
+  if (n_in < 1 || n_in > p->nS || n_out < 1 || n_out > p->nS)
+    return set_err(NLM_INVALID_ARGUMENT, "invalid number of input / output modes");
+  if (int e = check_pumps_ready(p)) return e;
+  const int N = p->N;
+  const size_t W = (size_t)n_out * N, nF = 2 * (size_t)n_in * N;
+  if (int e = p->G.ensure(nF * W)) return e;
+  if (int e = p->C.ensure(W * n_in * N)) return e;
+  if (int e = p->S.ensure(W * n_in * N)) return e;
+  const int nb = n_in < n_out ? n_in : n_out;
+  DevBuf<cplx> scale;
+  KParams kp;
+  fill_common(p, kp);
+  kp.kind = KIND_GREENS;
+  kp.nFields = (int)nF;
+  kp.inTime = in_time;
+  kp.outTime = in_time;
+  kp.nIn = n_in;
+  kp.nOut = n_out;
+  kp.out = p->G.p;
+  for (int i = 0; i < n_in; i++) {
+    if (inputs[i] < 0 || inputs[i] >= p->nS) return set_err(NLM_INVALID_ARGUMENT, "input mode out of range");
+    kp.inputs[i] = inputs[i];
+  }
+  for (int om = 0; om < n_out; om++) {
+    if (outputs[om] < 0 || outputs[om] >= p->nS) return set_err(NLM_INVALID_ARGUMENT, "output mode out of range");
+    kp.outPos[outputs[om]] = om;
+  }
+  if (block_scale) {
+    if (int e = scale.ensure((size_t)nb * n_out)) return e;
+    HIP_TRY(hipMemcpyAsync(scale.p, block_scale, sizeof(cplx) * nb * n_out, hipMemcpyHostToDevice, p->stream));
+  }
+  tic(p, 2);
+  if (int e = launch_prop_kernel(p, kp)) {
+    scale.release();
+    return e;
+  }
+  tic(p, 3);
+  const long total = (long)W * n_in * N;
+  hipLaunchKernelGGL(k_greens_assemble, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, p->stream, p->G.p, p->C.p,
+                     p->S.p, N, n_in, n_out, nb, block_scale ? scale.p : nullptr);
+  HIP_TRY(hipGetLastError());
+  tic(p, 4);
+  p->greensRows = (int)W;
+  p->greensCols = n_in * N;
+  if (C) HIP_TRY(hipMemcpyAsync(C, p->C.p, total * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  if (S) HIP_TRY(hipMemcpyAsync(S, p->S.p, total * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  scale.release();
+  if (p->timing) {
+    collect_timing(p, 2, 3, 1);
+    collect_timing(p, 3, 4, 2);
+  }
+  return NLM_OK;
+}
+
+extern "C" int nlm_greens_download(nlm_plan* p, double* C, double* S) {
+  if (!p) return set_err(NLM_INVALID_ARGUMENT, "null plan");
+  if (!p->greensRows) return set_err(NLM_RUNTIME_ERROR, "no Green's function has been computed");
+  const size_t total = (size_t)p->greensRows * p->greensCols;
+  if (C) HIP_TRY(hipMemcpyAsync(C, p->C.p, total * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  if (S) HIP_TRY(hipMemcpyAsync(S, p->S.p, total * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fft(int n, int count, const double* in, double* out, int inverse) {
+  int L;
+  if (int e = check_n(n, &L)) return e;
+  if (count < 0 || (count > 0 && (!in || !out))) return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
+  if (count == 0) return NLM_OK;
+  const cplx* tw = nullptr;
+  if (int e = get_twiddles(n, &tw)) return e;
+  DevBuf<cplx> a, b;
+  const size_t cnt = (size_t)n * count;
+  if (int e = a.ensure(cnt)) return e;
+  if (int e = b.ensure(cnt)) {
+    a.release();
+    return e;
+  }
+  hipError_t err = hipMemcpy(a.p, in, cnt * sizeof(cplx), hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    pick_fft(L)(a.p, b.p, count, tw, inverse != 0, 0);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipMemcpy(out, b.p, cnt * sizeof(cplx), hipMemcpyDeviceToHost);
+  a.release();
+  b.release();
+  if (err != hipSuccess) return set_err(NLM_DEVICE_ERROR, std::string("nlm_fft: ") + hipGetErrorString(err));
+  return NLM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Cascade composition (src/Cascade.cpp:128-129):
+//   C = C2 C1 + S2 conj(S1),  S = C2 S1 + S2 conj(C1)
+// C2, S2: M x K; C1, S1: K x Ncol; LDS-tiled complex FP64 products.
+// ---------------------------------------------------------------------------
+
+static constexpr int CT = 16;
+
+__global__ void __launch_bounds__(CT * CT) k_compose(const cplx* __restrict__ C2, const cplx* __restrict__ S2,
+                                                      const cplx* __restrict__ C1, const cplx* __restrict__ S1,
+                                                      cplx* __restrict__ C, cplx* __restrict__ S, int M, int K,
+                                                      int Nc) {
+  __shared__ cplx a1[CT][CT + 1], a2[CT][CT + 1], b1[CT][CT + 1], b2[CT][CT + 1];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int row = blockIdx.y * CT + ty, col = blockIdx.x * CT + tx;
+  cplx accC = mk(0, 0), accS = mk(0, 0);
+  for (int k0 = 0; k0 < K; k0 += CT) {
+    const int ka = k0 + tx, kb = k0 + ty;
+    a1[ty][tx] = (row < M && ka < K) ? C2[(size_t)row * K + ka] : mk(0, 0);
+    a2[ty][tx] = (row < M && ka < K) ? S2[(size_t)row * K + ka] : mk(0, 0);
+    b1[ty][tx] = (kb < K && col < Nc) ? C1[(size_t)kb * Nc + col] : mk(0, 0);
+    b2[ty][tx] = (kb < K && col < Nc) ? S1[(size_t)kb * Nc + col] : mk(0, 0);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CT; k++) {
+      const cplx x1 = a1[ty][k], x2 = a2[ty][k], y1 = b1[k][tx], y2 = b2[k][tx];
+      accC = accC + x1 * y1 + x2 * conj(y2);
+      accS = accS + x1 * y2 + x2 * conj(y1);
+    }
+    __syncthreads();
+  }
+  if (row < M && col < Nc) {
+    C[(size_t)row * Nc + col] = accC;
+    S[(size_t)row * Nc + col] = accS;
+  }
+}
+
+extern "C" int nlm_greens_compose(int M, int K, int Nc, const double* C2, const double* S2, const double* C1,
+                                  const double* S1, double* C, double* S) {
+  if (M < 0 || K < 0 || Nc < 0 || !C2 || !S2 || !C1 || !S1 || !C || !S)
+    return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
+  if (M == 0 || Nc == 0) return NLM_OK;
+  DevBuf<cplx> dC2, dS2, dC1, dS1, dC, dS;
+  const size_t a = (size_t)M * K, b = (size_t)K * Nc, c = (size_t)M * Nc;
+  int e = NLM_OK;
+  if ((e = dC2.ensure(a)) || (e = dS2.ensure(a)) || (e = dC1.ensure(b)) || (e = dS1.ensure(b)) || (e = dC.ensure(c)) ||
+      (e = dS.ensure(c)))
+    return e;
+  hipError_t err = hipSuccess;
+  auto h2d = [&](cplx* d, const double* h, size_t n) {
+    if (err == hipSuccess && n) err = hipMemcpy(d, h, n * sizeof(cplx), hipMemcpyHostToDevice);
+  };
+  h2d(dC2.p, C2, a);
+  h2d(dS2.p, S2, a);
+  h2d(dC1.p, C1, b);
+  h2d(dS1.p, S1, b);
+  if (err == hipSuccess) {
+    dim3 grid((Nc + CT - 1) / CT, (M + CT - 1) / CT), block(CT, CT);
+    hipLaunchKernelGGL(k_compose, grid, block, 0, 0, dC2.p, dS2.p, dC1.p, dS1.p, dC.p, dS.p, M, K, Nc);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipMemcpy(C, dC.p, c * sizeof(cplx), hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(S, dS.p, c * sizeof(cplx), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return set_err(NLM_DEVICE_ERROR, std::string("nlm_greens_compose: ") + hipGetErrorString(err));
+  return NLM_OK;
+}

@@ -1,0 +1,189 @@
+// nlm_fft.hpp -- complex FP64 FFT for one field held by a group of lanes.
+//
+// A field of N = 2^L points is held by T = N/8 lanes, eight points per lane in
+// the "stride layout": lane t holds x[t + T*m] for m = 0..7.  The forward and
+// inverse transforms take the field in this layout and return it in the same
+// layout (frequency index k = t + T*m), so the pointwise work of a split step
+// (RK4 stages, dispersion phase, Green's impulse injection) never moves data.
+//
+// Algorithm: Cooley-Tukey decimation in frequency over the radix sequence
+// 8, (2|4), 8, 8, ...  Stage 0 works on the eight points a lane already holds;
+// every later stage exchanges points through LDS.  After stage s the data are
+// stored sub-array-major, sub-array g (digits c_1..c_s of the output index,
+// least significant first) at LDS index g*M_s + b.  The last stage's outputs
+// land in natural order, which is the stride layout again.  LDS indices are
+// padded by one element per eight so that the 16-byte accesses of eight
+// consecutive lanes fall in distinct banks.
+//
+// Conventions follow the reference's Eigen wrapper (src/CustomEigenFFT.h:54-137):
+// forward X_k = sum_j x_j exp(-2 pi i jk/N) unscaled; the inverse uses
+// exp(+2 pi i jk/N) and its 1/N factor is applied by the caller.
+#pragma once
+#include <hip/hip_runtime.h>
+
+struct __align__(16) cplx {
+  double re, im;
+};
+
+__host__ __device__ __forceinline__ cplx mk(double re, double im) { return cplx{re, im}; }
+__host__ __device__ __forceinline__ cplx operator+(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
+__host__ __device__ __forceinline__ cplx operator-(cplx a, cplx b) { return cplx{a.re - b.re, a.im - b.im}; }
+__host__ __device__ __forceinline__ cplx operator*(cplx a, cplx b) {
+  return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ __forceinline__ cplx operator*(double s, cplx a) { return cplx{s * a.re, s * a.im}; }
+__host__ __device__ __forceinline__ cplx operator*(cplx a, double s) { return cplx{a.re * s, a.im * s}; }
+__host__ __device__ __forceinline__ cplx conj(cplx a) { return cplx{a.re, -a.im}; }
+__host__ __device__ __forceinline__ double abs2(cplx a) { return a.re * a.re + a.im * a.im; }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ cplx mul_mi(cplx a) {
+  return INV ? cplx{-a.im, a.re} : cplx{a.im, -a.re};
+}
+
+template <int L>
+struct FftShape {
+  static constexpr int N = 1 << L;
+  static constexpr int T = N / 8;                       // lanes per field
+  static constexpr int S = L / 3 + ((L % 3) ? 1 : 0);   // number of stages
+  static constexpr int radix(int s) { return (s == 1 && (L % 3)) ? (1 << (L % 3)) : 8; }
+  static constexpr int before(int s) {                  // product of radices of stages < s
+    int p = 1;
+    for (int i = 0; i < s; i++) p *= radix(i);
+    return p;
+  }
+  static constexpr int lds_elems = N + N / 8;            // padded LDS footprint (cplx)
+};
+
+__device__ __forceinline__ int lpad(int x) { return x + (x >> 3); }
+
+// In-place R-point DFT of v[o], v[o+1], ..., v[o+R-1]; outputs in natural order.
+template <int R, bool INV>
+__device__ __forceinline__ void dft_small(cplx* v) {
+  if constexpr (R == 2) {
+    cplx a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  } else if constexpr (R == 4) {
+    cplx s0 = v[0] + v[2], d0 = v[0] - v[2], s1 = v[1] + v[3], d1 = mul_mi<INV>(v[1] - v[3]);
+    v[0] = s0 + s1;
+    v[2] = s0 - s1;
+    v[1] = d0 + d1;
+    v[3] = d0 - d1;
+  } else {
+    constexpr double r = 0.70710678118654752440;
+    cplx u0 = v[0] + v[4], w0 = v[0] - v[4];
+    cplx u1 = v[1] + v[5], w1 = v[1] - v[5];
+    cplx u2 = v[2] + v[6], w2 = v[2] - v[6];
+    cplx u3 = v[3] + v[7], w3 = v[3] - v[7];
+    // w_a *= W_8^a (forward W_8 = exp(-i pi/4))
+    if (!INV) {
+      w1 = cplx{(w1.re + w1.im) * r, (w1.im - w1.re) * r};
+      w3 = cplx{(w3.im - w3.re) * r, -(w3.re + w3.im) * r};
+    } else {
+      w1 = cplx{(w1.re - w1.im) * r, (w1.re + w1.im) * r};
+      w3 = cplx{-(w3.re + w3.im) * r, (w3.re - w3.im) * r};
+    }
+    w2 = mul_mi<INV>(w2);
+    cplx a0 = u0 + u2, a1 = u0 - u2, a2 = u1 + u3, a3 = mul_mi<INV>(u1 - u3);
+    cplx b0 = w0 + w2, b1 = w0 - w2, b2 = w1 + w3, b3 = mul_mi<INV>(w1 - w3);
+    v[0] = a0 + a2;
+    v[4] = a0 - a2;
+    v[2] = a1 + a3;
+    v[6] = a1 - a3;
+    v[1] = b0 + b2;
+    v[5] = b0 - b2;
+    v[3] = b1 + b3;
+    v[7] = b1 - b3;
+  }
+}
+
+template <bool INV>
+__device__ __forceinline__ cplx twmul(cplx x, cplx w) {
+  return INV ? x * conj(w) : x * w;
+}
+
+// Per-lane twiddles, fixed for the whole propagation: tw[s][j*R + c] for the
+// twiddled stages s = 0..S-2 (the last stage needs none).
+template <int L>
+struct FftTwiddles {
+  cplx w[FftShape<L>::S > 1 ? FftShape<L>::S - 1 : 1][8];
+};
+
+template <int L>
+__device__ __forceinline__ void fft_load_twiddles(FftTwiddles<L>& tw, const cplx* __restrict__ table, int t) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T, S = F::S;
+#pragma unroll
+  for (int s = 0; s < S - 1; s++) {
+    const int R = F::radix(s), P = F::before(s), M = N / P / R, NB = 8 / R;
+#pragma unroll
+    for (int j = 0; j < 8; j++) tw.w[s][j] = mk(1.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const int b = (t + T * j) % M;
+#pragma unroll
+      for (int c = 1; c < R; c++) tw.w[s][j * R + c] = table[(b * c * P) & (N - 1)];
+    }
+  }
+}
+
+// Stages s >= 1 (recursive template so every index is a compile-time shape).
+template <int L, bool INV, int s>
+__device__ __forceinline__ void fft_stages(cplx (&v)[8], cplx* __restrict__ buf, int t, const FftTwiddles<L>& tw) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T, S = F::S;
+  constexpr int R = F::radix(s), P = F::before(s), Mp = N / P, M = Mp / R, NB = 8 / R;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    const int beta = t + T * j;
+    const int b = beta % M, g = beta / M;
+#pragma unroll
+    for (int a = 0; a < R; a++) v[j * R + a] = buf[lpad(g * Mp + a * M + b)];
+  }
+#pragma unroll
+  for (int j = 0; j < NB; j++) dft_small<R, INV>(v + j * R);
+  if constexpr (s == S - 1) {
+    if constexpr (NB > 1) {
+      cplx o[8];
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+#pragma unroll
+        for (int c = 0; c < R; c++) o[j + NB * c] = v[j * R + c];
+#pragma unroll
+      for (int m = 0; m < 8; m++) v[m] = o[m];
+    }
+  } else {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const int beta = t + T * j;
+      const int b = beta % M, g = beta / M;
+#pragma unroll
+      for (int c = 0; c < R; c++) {
+        cplx x = c == 0 ? v[j * R] : twmul<INV>(v[j * R + c], tw.w[s][j * R + c]);
+        buf[lpad((g + P * c) * M + b)] = x;
+      }
+    }
+    fft_stages<L, INV, s + 1>(v, buf, t, tw);
+  }
+}
+
+// Transform the field (stride layout in, stride layout out).  `buf` is this
+// field's LDS region of FftShape<L>::lds_elems elements.  All lanes of the
+// workgroup must call this together (it contains workgroup barriers).
+template <int L, bool INV>
+__device__ __forceinline__ void fft_field(cplx (&v)[8], cplx* __restrict__ buf, int t, const FftTwiddles<L>& tw) {
+  using F = FftShape<L>;
+  constexpr int T = F::T, S = F::S;
+  dft_small<8, INV>(v);
+  if constexpr (S > 1) {
+#pragma unroll
+    for (int c = 1; c < 8; c++) v[c] = twmul<INV>(v[c], tw.w[0][c]);
+    __syncthreads();  // previous readers of buf are done
+#pragma unroll
+    for (int c = 0; c < 8; c++) buf[lpad(c * T + t)] = v[c];
+    fft_stages<L, INV, 1>(v, buf, t, tw);
+  }
+}

@@ -1,0 +1,154 @@
+"""Seeded solver configurations shared by the oracle and the engine tests.
+
+Each builder returns (class name, kwargs) given the module whose enums to use
+(the product's `nonlinearmedium` or the oracle), so the two sides are built
+from identical arguments.
+"""
+import math
+
+import numpy as np
+
+UNDEPLETED = ["Chi3", "Chi2PDC", "Chi2SFG", "Chi2AFC", "Chi2SFGPDC", "Chi2SFGII", "Chi2PDCII", "Chi2SFGOPA",
+              "Chi2SFGXPM"]
+DEPLETED = ["Chi2SHG", "Chi2SHGOPA", "Chi2DSFG", "Chi2SHGXPM", "Chi2ASHG", "Chi3GNLSE"]
+ALL = UNDEPLETED + DEPLETED
+NMODES = {"Chi3": 1, "Chi2PDC": 1, "Chi2SFG": 2, "Chi2AFC": 2, "Chi2SFGPDC": 2, "Chi2SFGII": 4, "Chi2PDCII": 2,
+          "Chi2SFGOPA": 2, "Chi2SFGXPM": 2, "Chi2SHG": 2, "Chi2SHGOPA": 4, "Chi2DSFG": 3, "Chi2SHGXPM": 2,
+          "Chi2ASHG": 2, "Chi3GNLSE": 1}
+
+
+def small_case(name, mod, N=64, nZ=48, seed=0, profile="GaussianBeam", poled=True):
+    """Random but well-conditioned parameters for solver `name`."""
+    r = np.random.default_rng(seed)
+    u = lambda a, b: float(r.uniform(a, b))
+    IP = mod.IntensityProfile
+    prof = getattr(IP, profile)
+    zR = u(0.8, 2.0) if profile != "Constant" else math.inf
+    tMax = 10.0
+    pol = list(r.uniform(0.5, 1.5, size=7)) if poled else []
+    common = dict(tMax=tMax, tPrecision=N, nZSteps=nZ)
+    if name == "Chi3":
+        return name, dict(relativeLength=u(0.5, 1.5), nlLength=1.0, beta2=-1.0, beta3=u(-0.1, 0.1),
+                          pulseType=mod.PulseType.Sech, rayleighLength=zR, intensityProfile=prof, **common)
+    if name == "Chi2PDC":
+        return name, dict(relativeLength=1.0, nlLength=u(0.6, 1.4), beta2=u(-1, 1), beta2s=u(-1, 1),
+                          beta1=u(-1, 1), beta1s=u(-1, 1), beta3=u(-.1, .1), beta3s=u(-.1, .1),
+                          diffBeta0=u(-3, 3), rayleighLength=zR, intensityProfile=prof, poling=pol, **common)
+    if name in ("Chi2SFG", "Chi2PDCII", "Chi2SFGPDC", "Chi2AFC"):
+        names = {"Chi2SFG": ("nlLength", "nlLengthOrig"), "Chi2PDCII": ("nlLengthI", "nlLengthII"),
+                 "Chi2SFGPDC": ("nlLength", "nlLengthOrig"), "Chi2AFC": ("nlLength", "nlLengthOrig")}[name]
+        kw = dict(relativeLength=1.0, beta2=u(-1, 1), beta2s=u(-1, 1), beta2o=u(-1, 1), beta1=u(-1, 1),
+                  beta1s=u(-1, 1), beta1o=u(-1, 1), beta3=u(-.1, .1), beta3s=u(-.1, .1), beta3o=u(-.1, .1),
+                  rayleighLength=zR, intensityProfile=prof, **common)
+        kw[names[0]] = u(0.6, 1.4)
+        kw[names[1]] = u(0.6, 1.4)
+        if name == "Chi2AFC":
+            kw.update(diffBeta0Start=u(-3, 0), diffBeta0End=u(0, 3))
+        else:
+            kw.update(diffBeta0=u(-3, 3), poling=pol)
+            if name == "Chi2SFGPDC":
+                kw["diffBeta0o"] = u(-3, 3)
+        return name, kw
+    if name == "Chi2SFGII":
+        # the reference binds "intensityProfile" to the zPrecision slot and vice versa
+        kw = dict(relativeLength=1.0, nlLengthZY=u(0.6, 1.4), nlLengthZZ=u(0.6, 1.4), beta2=u(-1, 1),
+                  beta2sz=u(-1, 1), beta2sy=u(-1, 1), beta2oz=u(-1, 1), beta2oy=u(-1, 1), beta1=u(-1, 1),
+                  beta1sz=u(-1, 1), beta1sy=u(-1, 1), beta1oz=u(-1, 1), beta1oy=u(-1, 1), diffBeta0z=u(-3, 3),
+                  diffBeta0y=u(-3, 3), diffBeta0=u(-3, 3), rayleighLength=zR, poling=pol,
+                  intensityProfile=100, zPrecision=prof, **common)
+        return name, kw
+    if name == "Chi2SFGOPA":
+        return name, dict(relativeLength=1.0, nlLengthSFGh=u(.6, 1.4), nlLengthSFGf=u(.6, 1.4),
+                          nlLengthDOPAh=u(.6, 1.4), nlLengthDOPAf=u(.6, 1.4), beta2F=u(-1, 1), beta2H=u(-1, 1),
+                          beta2h=u(-1, 1), beta2f=u(-1, 1), beta1F=u(-1, 1), beta1H=u(-1, 1), beta1h=u(-1, 1),
+                          beta1f=u(-1, 1), diffBeta0SFG=u(-3, 3), diffBeta0DOPA=u(-3, 3), rayleighLength=zR,
+                          intensityProfile=prof, poling=pol, **common)
+    if name == "Chi2SFGXPM":
+        return name, dict(relativeLength=1.0, nlLength=u(.6, 1.4), nlLengthOrig=u(.6, 1.4), nlLengthChi3=u(1, 3),
+                          nlLengthChi3Orig=u(1, 3), beta2=u(-1, 1), beta2s=u(-1, 1), beta2o=u(-1, 1),
+                          beta1=u(-1, 1), beta1s=u(-1, 1), beta1o=u(-1, 1), diffBeta0=u(-3, 3), rayleighLength=zR,
+                          intensityProfile=prof, poling=pol, **common)
+    if name == "Chi2SHG":
+        return name, dict(relativeLength=1.0, nlLengthH=u(.6, 1.4), nlLengthP=u(.6, 1.4), beta2h=u(-1, 1),
+                          beta2p=u(-1, 1), beta1h=u(-1, 1), beta1p=u(-1, 1), beta3h=u(-.1, .1), beta3p=u(-.1, .1),
+                          diffBeta0=u(-3, 3), rayleighLength=zR, intensityProfile=prof, poling=pol, **common)
+    if name == "Chi2SHGOPA":
+        return name, dict(relativeLength=1.0, nlLengthP=u(.6, 1.4), nlLengthSH=u(.6, 1.4), nlLengthPA1=u(.6, 1.4),
+                          nlLengthPA2=u(.6, 1.4), beta2p=u(-1, 1), beta2sh=u(-1, 1), beta2pa1=u(-1, 1),
+                          beta2pa2=u(-1, 1), beta1p=u(-1, 1), beta1sh=u(-1, 1), beta1pa1=u(-1, 1), beta1pa2=u(-1, 1),
+                          diffBeta0shg=u(-3, 3), diffBeta0opa=u(-3, 3), rayleighLength=zR, intensityProfile=prof,
+                          poling=pol, **common)
+    if name == "Chi2DSFG":
+        return name, dict(relativeLength=1.0, nlLengthP=u(.6, 1.4), nlLengthS=u(.6, 1.4), nlLengthD=u(.6, 1.4),
+                          beta2p=u(-1, 1), beta2s=u(-1, 1), beta2d=u(-1, 1), beta1p=u(-1, 1), beta1s=u(-1, 1),
+                          beta1d=u(-1, 1), diffBeta0=u(-3, 3), rayleighLength=zR, intensityProfile=prof, poling=pol,
+                          **common)
+    if name == "Chi2SHGXPM":
+        return name, dict(relativeLength=1.0, nlLengthH=u(.6, 1.4), nlLengthP=u(.6, 1.4), nlLengthChi3=u(1, 3),
+                          beta2h=u(-1, 1), beta2p=u(-1, 1), beta1h=u(-1, 1), beta1p=u(-1, 1), diffBeta0=u(-3, 3),
+                          rayleighLength=zR, intensityProfile=prof, poling=pol, **common)
+    if name == "Chi2ASHG":
+        return name, dict(relativeLength=1.0, nlLengthH=u(.6, 1.4), nlLengthP=u(.6, 1.4), beta2h=u(-1, 1),
+                          beta2p=u(-1, 1), beta1h=u(-1, 1), beta1p=u(-1, 1), diffBeta0Start=u(-3, 0),
+                          diffBeta0End=u(0, 3), rayleighLength=zR, intensityProfile=prof, **common)
+    if name == "Chi3GNLSE":
+        return name, dict(relativeLength=u(0.3, 0.8), nlLength=1.0, selfSteepLength=u(20, 100), fr=0.18, fb=0.21,
+                          tau1=0.0122 * 20, tau2=0.032 * 20, tau3=0.096 * 20, beta2=-1.0, beta3=u(-.05, .05),
+                          rayleighLength=zR, intensityProfile=prof, **common)
+    raise KeyError(name)
+
+
+def make(mod, name, kw):
+    obj = getattr(mod, name)(**kw)
+    if name == "Chi2SFGOPA":
+        # second pump: a delayed Gaussian
+        tau = np.asarray(obj.tau)
+        obj.setPump(np.exp(-0.5 * (tau - 1.0) ** 2) * 0.7, 0, 0, 1)
+    return obj
+
+
+def random_inputs(name, N, rows, seed, nch=None):
+    r = np.random.default_rng(seed)
+    nch = nch or NMODES[name]
+    shape = (rows, nch * N)
+    x = (r.standard_normal(shape) + 1j * r.standard_normal(shape)) * 0.3
+    if name in ("Chi3GNLSE",):
+        x *= 0.5
+    return x
+
+
+# ---------------------------------------------------------------------------
+# the configurations quoted by the benchmark definition
+# ---------------------------------------------------------------------------
+
+SEED0 = 20261202
+
+
+def config1(mod):
+    """Chi3 soliton pump N=512, tMax=10, z=pi/2 (zPrecision 637 -> nZ=1000), Gaussian signal."""
+    fiber = mod.Chi3(relativeLength=math.pi / 2, nlLength=1, beta2=-1, beta3=0, pulseType=mod.PulseType.Sech,
+                     tMax=10, tPrecision=512, zPrecision=637)
+    r = np.random.default_rng(SEED0 + 1)
+    phi = r.uniform(0, 2 * math.pi)
+    tau = np.asarray(fiber.tau)
+    sig = np.exp(-tau ** 2 / 2) * np.exp(1j * phi)
+    return fiber, sig
+
+
+def config2_kwargs():
+    r = np.random.default_rng(SEED0 + 2)
+    w = r.uniform(0.5, 2)
+    b1, b1s = r.uniform(-2, 2, size=2)
+    b2, b2s = r.uniform(-1, 1, size=2)
+    d0 = r.uniform(-5, 5)
+    return dict(relativeLength=1.0, nlLength=1.0, beta2=float(b2), beta2s=float(b2s), beta1=float(b1),
+                beta1s=float(b1s), diffBeta0=float(d0), tMax=20.0, tPrecision=512, nZSteps=1000), float(w)
+
+
+def config2(mod, N=512, nZ=1000):
+    """Chi2PDC Green's function: N=512, tMax=20, nZ=1000, seeded Gaussian pump and dispersion."""
+    kw, w = config2_kwargs()
+    kw = dict(kw, tPrecision=N, nZSteps=nZ)
+    tau = 2 * kw["tMax"] / N * np.fft.fftshift(np.arange(-N // 2, N // 2))
+    kw["customPump"] = np.exp(-tau ** 2 / (2 * w * w)).astype(np.complex128)
+    return mod.Chi2PDC(**kw)
