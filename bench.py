@@ -1,0 +1,152 @@
+"""Benchmark: batched Green's-function solve of a Chi2PDC medium on one MI355X.
+
+Workload (the benchmark definition's Green's configuration): Chi2PDC, nFreqs=512,
+tMax=20, nZSteps=1000, seeded Gaussian pump (width ~ U(0.5, 2)) and dispersion
+(beta1, beta1s ~ U(-2, 2); beta2, beta2s ~ U(-1, 1); diffBeta0 ~ U(-5, 5)),
+uniform poling.  One step = one computeGreensFunction(): the pump grid
+(2nZ-1 rows), the 2N impulse propagations of nZ-1 split steps each in one
+launch, and the C/S assembly, with C and S left in HBM (the host copy is not
+timed).  Metric: field-steps/s, a field-step being one propagation advanced by
+one RK4 step plus one dispersion step (2*512*999 per solve).
+
+Reported beside it:
+  plain      the same kernel run as one propagation per launch, serially
+             (sampled on --plain-sample impulses, scaled to the full solve);
+  roofline   the propagation kernel's achieved FP64 rate (algorithmic flops
+             10 N log2 N + 80 N per field-step) against the 78.6 TFLOP/s peak,
+             from HIP events on the kernel's own stream;
+  cpu_baseline  the C oracle (a restatement of the reference C++ path with a
+             kissfft-class FFT and the reference's std::thread fan-out) timed on
+             this host on the same Green's solve;
+  rel_err    relative L2 difference of (C, S) against that CPU result.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(HERE, "tests"))
+
+PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 (vector and matrix) peak
+HBM_PEAK_GBS = 8000.0
+N_FREQS, NZ = 512, 1000
+
+
+def flops_per_field_step(n):
+    """Algorithmic FP64 flops of one Chi2PDC field-step: two FFTs (5 N log2 N each),
+    the RK4 stages (72 N), the dispersion phase (6 N) and the inverse scale (2 N)."""
+    return 10 * n * math.log2(n) + 80 * n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-sample", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (default: min(16, cores))")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the split-step engine on a single MI355X "
+              "(no multi-GPU path); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import multimodenonlinearoptics_amd as pkg
+    from cases import config2
+
+    nlm = pkg.nonlinearmedium
+    medium = config2(nlm, N_FREQS, NZ)
+    medium.enableTiming(True)
+    field_steps = 2 * N_FREQS * (NZ - 1)
+
+    for _ in range(args.warmup):
+        medium.greensOnDevice()
+    medium.sync()
+
+    prop_ms, pump_ms, asm_ms = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        medium.greensOnDevice()
+        tp = medium.lastTiming()
+        pump_ms.append(tp[0])
+        prop_ms.append(tp[1])
+        asm_ms.append(tp[2])
+    medium.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = field_steps / (ms_per_step * 1e-3)
+
+    prop_avg = sum(prop_ms) / len(prop_ms)
+    fl = flops_per_field_step(N_FREQS)
+    achieved_tflops = fl * field_steps / (prop_avg * 1e-3) / 1e12
+
+    # plain: one propagation per launch, serially (same kernel)
+    medium.runPumpSimulation()
+    tau = np.asarray(medium.tau)
+    sample = max(1, args.plain_sample)
+    X = np.zeros((1, N_FREQS), dtype=np.complex128)
+    medium.batchSignalSimulation(X, runPump=False)  # warm
+    tp0 = time.perf_counter()
+    for j in range(sample):
+        X[:] = 0
+        X[0, j % N_FREQS] = 1.0 if j % 2 == 0 else 1j
+        medium.batchSignalSimulation(X, runPump=False)
+    tp1 = time.perf_counter()
+    plain_per_prop = (tp1 - tp0) / sample
+    plain_solve_s = plain_per_prop * 2 * N_FREQS
+    plain_value = field_steps / plain_solve_s
+
+    rec = {
+        "metric": "field-steps/s FP64 (rel.err<=1e-10), batched Green's solve vs plain GPU",
+        "value": value,
+        "unit": "field-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian pump and dispersion, impulse inputs)",
+        "config": {"workload": "Chi2PDC computeGreensFunction, nFreqs=512, tMax=20, nZSteps=1000, "
+                               "2x512 impulse propagations per solve",
+                   "nFreqs": N_FREQS, "nZSteps": NZ, "field_steps_per_step": field_steps},
+        "kernel_ms": {"pump": sum(pump_ms) / len(pump_ms), "propagation": prop_avg,
+                      "assembly": sum(asm_ms) / len(asm_ms)},
+        "plain": {"value": plain_value, "unit": "field-steps/s", "sample_propagations": sample,
+                  "s_per_solve": plain_solve_s},
+        "batched_vs_plain": value / plain_value,
+        "grad_vs_forward": None,
+        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "k_prop<MChi2PDC, 9>", "flops_per_field_step": fl},
+    }
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ref = config2(O, N_FREQS, NZ)
+        tc0 = time.perf_counter()
+        Co, So = ref.computeGreensFunction(nThreads=threads)
+        tc1 = time.perf_counter()
+        cpu_value = field_steps / (tc1 - tc0)
+        C, S = medium.computeGreensFunction()
+        num = np.linalg.norm(np.hstack([C - Co, S - So]))
+        den = np.linalg.norm(np.hstack([Co, So]))
+        rec["cpu_baseline"] = {"value": cpu_value, "unit": "field-steps/s", "cores": threads, "kind": "port",
+                               "sample": "one full Green's solve (2x512 propagations x 999 steps) on %d threads, "
+                                         "%.1f s" % (threads, tc1 - tc0)}
+        rec["vs_cpu"] = value / cpu_value
+        rec["rel_err"] = float(num / den)
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

@@ -520,6 +520,29 @@ static std::vector<int32_t> select_modes(int nS, const std::vector<uint8_t>& use
 std::pair<Grid, Grid> Medium::computeGreensFunction(bool inTimeDomain, bool runPump, unsigned nThreads, bool normalize,
                                                     const std::vector<uint8_t>& useInput,
                                                     const std::vector<uint8_t>& useOutput) {
+  return greensImpl(inTimeDomain, runPump, nThreads, normalize, useInput, useOutput, true);
+}
+
+void Medium::greensOnDevice(bool inTimeDomain, bool runPump) {
+  if (fullyNonlinear()) throw std::runtime_error("Object does not have this method.");
+  greensImpl(inTimeDomain, runPump, 1, false, {}, {}, false);
+}
+
+void Medium::enableTiming(bool on) { check(nlm_plan_enable_timing(plan(), on ? 1 : 0)); }
+
+std::vector<double> Medium::lastTiming() {
+  std::vector<double> ms(3, 0.0);
+  if (plan_) check(nlm_plan_last_timing(plan_, ms.data()));
+  return ms;
+}
+
+void Medium::sync() {
+  if (plan_) check(nlm_plan_sync(plan_));
+}
+
+std::pair<Grid, Grid> Medium::greensImpl(bool inTimeDomain, bool runPump, unsigned nThreads, bool normalize,
+                                         const std::vector<uint8_t>& useInput, const std::vector<uint8_t>& useOutput,
+                                         bool download) {
   if (useInput.size() > (size_t)nS_)
     throw std::invalid_argument("List of requested inputs indices longer than number of modes!");
   if (useOutput.size() > (size_t)nS_)
@@ -549,11 +572,14 @@ std::pair<Grid, Grid> Medium::computeGreensFunction(bool inTimeDomain, bool runP
         if (im != om) scale[(size_t)im * nOut + om] = std::sqrt(nlStep_[im] / nlStep_[om]);
   }
   std::pair<Grid, Grid> res;
-  res.first.setZero(nOut * N_, nIn * N_);
-  res.second.setZero(nOut * N_, nIn * N_);
+  if (download) {
+    res.first.setZero(nOut * N_, nIn * N_);
+    res.second.setZero(nOut * N_, nIn * N_);
+  }
   check(nlm_greens(p, inTimeDomain ? 1 : 0, nIn, ins.data(), nOut, outs.data(),
                    normalize ? reinterpret_cast<const double*>(scale.data()) : nullptr,
-                   reinterpret_cast<double*>(res.first.v.data()), reinterpret_cast<double*>(res.second.v.data())));
+                   download ? reinterpret_cast<double*>(res.first.v.data()) : nullptr,
+                   download ? reinterpret_cast<double*>(res.second.v.data()) : nullptr));
   return res;
 }
 

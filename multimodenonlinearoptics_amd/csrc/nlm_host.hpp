@@ -92,6 +92,12 @@ class Medium {
   // the device plan (created on first use); exposed for timing and streams
   nlm_plan* plan();
   void setStream(void* hipStream);
+  // Green's function with C and S left in device memory (no host copy); the
+  // default mode selection of computeGreensFunction.  For timing the engine.
+  void greensOnDevice(bool inTimeDomain, bool runPump);
+  void enableTiming(bool on);
+  std::vector<double> lastTiming();
+  void sync();
 
   const std::vector<cd>& raman() const { return raman_; }
 
@@ -133,6 +139,9 @@ class Medium {
   double relativeIntensity(double i) const;
   static std::vector<double> linSpaced(int n, double low, double high);
 
+  std::pair<Grid, Grid> greensImpl(bool inTimeDomain, bool runPump, unsigned nThreads, bool normalize,
+                                   const std::vector<uint8_t>& useInput, const std::vector<uint8_t>& useOutput,
+                                   bool download);
   void markPumpOnDevice();
   void syncPumpToHost(unsigned i);
   void syncSignalToHost();

@@ -225,6 +225,20 @@ PYBIND11_MODULE(nonlinearmedium, m) {
            "Set the per-step poling value directly (length nZSteps; extension).", "profile"_a);
   base.def("setStream", [](Medium& md, std::uintptr_t s) { md.setStream(reinterpret_cast<void*>(s)); },
            "Run this medium's device work on the given hipStream_t (0: the medium's own stream).", "stream"_a);
+  base.def("greensOnDevice",
+           [](Medium& md, bool inTimeDomain, bool runPump) {
+             py::gil_scoped_release nogil;
+             md.greensOnDevice(inTimeDomain, runPump);
+           },
+           "Compute the Green's function and keep C and S in device memory (extension, for timing).",
+           "inTimeDomain"_a = false, "runPump"_a = true);
+  base.def("enableTiming", &Medium::enableTiming, "Record HIP-event kernel times of each run (extension).", "on"_a);
+  base.def("lastTiming", &Medium::lastTiming,
+           "Kernel times of the last run in ms: [pump, propagation, assembly] (extension).");
+  base.def("sync", [](Medium& md) {
+    py::gil_scoped_release nogil;
+    md.sync();
+  }, "Wait for this medium's device work (extension).");
 
   py::enum_<nlm::PulseType>(m, "PulseType")
       .value("Gaussian", nlm::PulseType::Gaussian)
