@@ -32,6 +32,10 @@ sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.join(HERE, "tests"))
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 (vector and matrix) peak
+# HBM bytes per Green's launch of the propagation kernel: rocprofv3 FETCH_SIZE and
+# WRITE_SIZE, each scaled by the counter/actual ratio of a 256 MiB copy kernel
+# (x2 and x1); profiles/k_prop_chi2pdc_greens_n512_nz1000.md
+TRAFFIC_BYTES_PER_LAUNCH = 1.40e8
 HBM_PEAK_GBS = 8000.0
 N_FREQS, NZ = 512, 1000
 
@@ -50,7 +54,19 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (default: min(16, cores))")
+    ap.add_argument("--copy-probe-mb", type=float, default=0.0,
+                    help="only run a device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe_mb > 0:
+        import torch
+        n = int(args.copy_probe_mb * (1 << 20)) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 8}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the split-step engine on a single MI355X "
               "(no multi-GPU path); run with --gpus 1", file=sys.stderr)
@@ -89,7 +105,7 @@ def main():
     # plain: one propagation per launch, serially (same kernel)
     medium.runPumpSimulation()
     tau = np.asarray(medium.tau)
-    sample = max(1, args.plain_sample)
+    sample = max(0, args.plain_sample)
     X = np.zeros((1, N_FREQS), dtype=np.complex128)
     medium.batchSignalSimulation(X, runPump=False)  # warm
     tp0 = time.perf_counter()
@@ -98,9 +114,9 @@ def main():
         X[0, j % N_FREQS] = 1.0 if j % 2 == 0 else 1j
         medium.batchSignalSimulation(X, runPump=False)
     tp1 = time.perf_counter()
-    plain_per_prop = (tp1 - tp0) / sample
+    plain_per_prop = (tp1 - tp0) / max(sample, 1)
     plain_solve_s = plain_per_prop * 2 * N_FREQS
-    plain_value = field_steps / plain_solve_s
+    plain_value = field_steps / plain_solve_s if sample else None
 
     rec = {
         "metric": "field-steps/s FP64 (rel.err<=1e-10), batched Green's solve vs plain GPU",
@@ -122,10 +138,10 @@ def main():
                       "assembly": sum(asm_ms) / len(asm_ms)},
         "plain": {"value": plain_value, "unit": "field-steps/s", "sample_propagations": sample,
                   "s_per_solve": plain_solve_s},
-        "batched_vs_plain": value / plain_value,
+        "batched_vs_plain": value / plain_value if plain_value else None,
         "grad_vs_forward": None,
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
                      "kernel": "k_prop<MChi2PDC, 9>", "flops_per_field_step": fl},
     }
 

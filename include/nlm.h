@@ -144,6 +144,24 @@ int nlm_batch_dev(nlm_plan* plan, const double* inputs_dev, int rows, int cols, 
 int nlm_plan_enable_timing(nlm_plan* plan, int enable);
 int nlm_plan_last_timing(nlm_plan* plan, double ms[3]);
 
+/* Differentiable Green's function of a single-mode Chi2PDC medium (linear in
+ * the pump, conjugate-coupled signal; computeGreensFunction,
+ * src/_NonlinearMedium.cpp:263-375, with Chi2PDC::DiffEq,
+ * solvers/Chi2PDC.cpp:27-47).  nlm_greens_forward is nlm_greens with
+ * time-domain field checkpoints every `ckpt_every` z steps (<= 0: ceil(sqrt(nZ-1)))
+ * kept on the device.  nlm_greens_adjoint then takes dL/dC, dL/dS (complex,
+ * convention dL/dRe + i dL/dIm, the layout of C and S) and returns
+ *   g_coef          [nZ][3][n_step_coef] gradient w.r.t. the step coefficients,
+ *   g_pump_time     [2nZ-1][N] gradient w.r.t. the pump time grid (or NULL),
+ *   g_envelope      [N] gradient w.r.t. the time-domain pump envelope through
+ *                   the linear pump grid (nlm_pump_linear must have built it; or NULL).
+ * The reverse sweep recomputes each checkpoint segment; reductions are done in
+ * a fixed order (no atomics). */
+int nlm_greens_forward(nlm_plan* plan, int in_time_domain, int n_in, const int32_t* inputs, int n_out,
+                       const int32_t* outputs, const double* block_scale, int ckpt_every, double* C, double* S);
+int nlm_greens_adjoint(nlm_plan* plan, const double* gC, const double* gS, double* g_coef, double* g_pump_time,
+                       double* g_envelope);
+
 /* Green's function composition of two media, Cascade::computeGreensFunction
  * (src/Cascade.cpp:128-129): C = C2 C1 + S2 conj(S1), S = C2 S1 + S2 conj(C1).
  * C2, S2: M x K; C1, S1: K x Ncol; C, S: M x Ncol (host, row-major). */

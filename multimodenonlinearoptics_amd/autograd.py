@@ -1,0 +1,48 @@
+"""PyTorch autograd over the MI355X engine's adjoint Green's function.
+
+``GreensFunction.apply(medium, envelope, poling)`` returns the Green's
+matrices (C, S) of a single-mode Chi2PDC medium as complex128 tensors, for a
+time-domain pump envelope (complex, length nFreqs) and a per-z-step poling
+profile (real, length nZSteps).  Its backward runs the device adjoint (forward
+checkpoints every ~sqrt(nZ) steps, recomputation between them) and returns
+dL/d envelope and dL/d poling in torch's complex-gradient convention.
+
+The forward replaces computeGreensFunction (src/_NonlinearMedium.cpp:263-375)
+for a Chi2PDC medium (solvers/Chi2PDC.cpp:27-47) after setPump(customPump)
+(src/_NonlinearMedium.cpp:207-222) and a per-step poling.
+"""
+import numpy as np
+import torch
+
+
+class GreensFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, medium, envelope, poling, ckpt_every=0):
+        env = envelope.detach().cpu().numpy().astype(np.complex128)
+        pol = poling.detach().cpu().numpy().astype(np.float64)
+        medium.setPump(env)
+        medium.setPolingProfile(pol)
+        C, S = medium.greensForward(False, True, int(ckpt_every))
+        ctx.medium = medium
+        ctx.device = envelope.device
+        return torch.from_numpy(C).to(envelope.device), torch.from_numpy(S).to(envelope.device)
+
+    @staticmethod
+    def backward(ctx, gC, gS):
+        N = ctx.medium.nZSteps  # noqa: F841  (kept for clarity of the shapes below)
+        gC = np.zeros(0) if gC is None else gC.detach().cpu().numpy()
+        gS = np.zeros(0) if gS is None else gS.detach().cpu().numpy()
+        n = ctx.medium.signalFreq.shape[1]
+        if gC.size == 0:
+            gC = np.zeros((n, n), dtype=np.complex128)
+        if gS.size == 0:
+            gS = np.zeros((n, n), dtype=np.complex128)
+        g_env, g_pol = ctx.medium.greensAdjoint(gC.astype(np.complex128), gS.astype(np.complex128))
+        return (None, torch.from_numpy(g_env).to(ctx.device), torch.from_numpy(g_pol).to(ctx.device), None)
+
+
+def squeezing_loss(S, r=1.5):
+    """||S S^T - r I||_F^2 (the differentiable-poling benchmark loss)."""
+    n = S.shape[0]
+    M = S @ S.transpose(0, 1) - r * torch.eye(n, dtype=S.dtype, device=S.device)
+    return (M.abs() ** 2).sum()

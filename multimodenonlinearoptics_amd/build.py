@@ -57,9 +57,14 @@ def build(force=False, jobs=None, verbose=False):
     objs = []
     o = os.path.join(OBJ, "nlm_device.o")
     objs.append(o)
-    deps = _deps("nlm_device.hip", *dev_hdrs)
+    deps = _deps("nlm_device.hip", "nlm_internal.hpp", *dev_hdrs)
     if force or _stale(o, deps):
         steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_device.hip"), "-o", o])
+    o = os.path.join(OBJ, "nlm_adjoint.o")
+    objs.append(o)
+    deps = _deps("nlm_adjoint.hip", "nlm_adjoint.hpp", "nlm_internal.hpp", *dev_hdrs)
+    if force or _stale(o, deps):
+        steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_adjoint.hip"), "-o", o])
     for m in PROP_MODELS:
         o = os.path.join(OBJ, "nlm_prop_%d.o" % m)
         objs.append(o)

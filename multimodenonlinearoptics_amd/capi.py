@@ -15,7 +15,7 @@ SYMBOLS = [
     "nlm_plan_set_coef", "nlm_pump_linear", "nlm_pump_chi3", "nlm_pump_upload", "nlm_pump_download",
     "nlm_propagate_grid", "nlm_grid_download", "nlm_batch", "nlm_greens", "nlm_greens_download",
     "nlm_batch_dev", "nlm_plan_enable_timing", "nlm_plan_last_timing", "nlm_greens_compose", "nlm_fft",
-    "nlm_plan_sync", "nlm_last_error", "nlm_version",
+    "nlm_plan_sync", "nlm_last_error", "nlm_version", "nlm_greens_forward", "nlm_greens_adjoint",
 ]
 
 _lib = None

@@ -26,7 +26,7 @@
 #include <vector>
 
 #include "../../include/nlm.h"
-#include "nlm_prop.hpp"
+#include "nlm_internal.hpp"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -34,17 +34,10 @@
 
 static thread_local std::string g_err;
 
-static int set_err(int kind, const std::string& msg) {
+int set_err(int kind, const std::string& msg) {
   g_err = msg;
   return kind;
 }
-
-#define HIP_TRY(expr)                                                                              \
-  do {                                                                                             \
-    hipError_t e_ = (expr);                                                                        \
-    if (e_ != hipSuccess)                                                                          \
-      return set_err(NLM_DEVICE_ERROR, std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
 extern "C" const char* nlm_last_error(void) { return g_err.c_str(); }
 extern "C" const char* nlm_version(void) { return "nlm-hip 0.1 (gfx950)"; }
@@ -214,6 +207,13 @@ __global__ void k_greens_assemble(const cplx* __restrict__ G, cplx* __restrict__
   S[idx] = sv;
 }
 
+void launch_greens_assemble(const cplx* G, cplx* C, cplx* S, int N, int nIn, int nOut, int nb, const cplx* scale,
+                            hipStream_t s) {
+  const long total = (long)nOut * N * nIn * N;
+  hipLaunchKernelGGL(k_greens_assemble, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, G, C, S, N, nIn, nOut,
+                     nb, scale);
+}
+
 // ---------------------------------------------------------------------------
 // dispatch over (model, log2 N)
 // ---------------------------------------------------------------------------
@@ -324,33 +324,11 @@ static std::vector<cplx> twiddle_table(int n) {
   return w;
 }
 
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  int ensure(size_t count) {
-    if (count <= n && p) return NLM_OK;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-    if (count == 0) return NLM_OK;
-    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-    if (e != hipSuccess) return set_err(NLM_DEVICE_ERROR, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-    n = count;
-    return NLM_OK;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-};
-
 // Twiddle tables shared by all plans of one device and size.
 static std::mutex g_tw_mu;
 static std::map<std::pair<int, int>, cplx*> g_tw;
 
-static int get_twiddles(int n, const cplx** out) {
+int get_twiddles(int n, const cplx** out) {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_tw_mu);
@@ -368,23 +346,6 @@ static int get_twiddles(int n, const cplx** out) {
   *out = d;
   return NLM_OK;
 }
-
-struct nlm_plan_s {
-  int model = 0, N = 0, L = 0, nZ = 0, nS = 0, nP = 0, NC = 0, NCC = 0;
-  int device = 0;
-  hipStream_t own = nullptr, stream = nullptr;
-  const cplx* tw = nullptr;
-  DevBuf<cplx> dstep, halfIn, halfOut, coef, cconst, raman, filt;
-  DevBuf<cplx> pumpT[2], pumpF[2];
-  DevBuf<cplx> gridF, gridT;
-  DevBuf<cplx> G, C, S, in, out, env, F0, dsp, relI;
-  DevBuf<double> Dp, strength;
-  int greensRows = 0, greensCols = 0;
-  bool timing = false;
-  hipEvent_t ev[6] = {};
-  double last_ms[3] = {0, 0, 0};
-  std::vector<cplx> host_tmp;
-};
 
 static const int kModelNS[NLM_NUM_MODELS] = {1, 1, 2, 2, 2, 4, 2, 2, 2, 2, 4, 3, 2, 2, 1};
 static const int kModelNP[NLM_NUM_MODELS] = {1, 1, 1, 1, 1, 1, 1, 2, 1, 0, 0, 0, 0, 0, 0};
@@ -407,7 +368,7 @@ static int upload(DevBuf<cplx>& b, const double* src, size_t count, hipStream_t 
   return NLM_OK;
 }
 
-static void tic(nlm_plan* p, int slot) {
+void tic(nlm_plan* p, int slot) {
   if (p->timing) (void)hipEventRecord(p->ev[slot], p->stream);
 }
 
@@ -466,6 +427,9 @@ extern "C" int nlm_plan_destroy(nlm_plan* p) {
                           &p->pumpT[0], &p->pumpT[1], &p->pumpF[0], &p->pumpF[1], &p->gridF, &p->gridT,
                           &p->G, &p->C, &p->S, &p->in, &p->out, &p->env, &p->F0, &p->dsp, &p->relI};
   for (auto* b : bufs) b->release();
+  DevBuf<cplx>* abufs[] = {&p->ckpt, &p->seg, &p->gG, &p->gC, &p->gS, &p->gcoefPart, &p->gcoef, &p->gpumpPart,
+                           &p->gpump, &p->gF0Part, &p->gF0, &p->gtmp, &p->scale};
+  for (auto* b : abufs) b->release();
   p->Dp.release();
   p->strength.release();
   for (auto& ev : p->ev)
@@ -506,7 +470,7 @@ extern "C" int nlm_plan_last_timing(nlm_plan* p, double ms[3]) {
   return NLM_OK;
 }
 
-static void collect_timing(nlm_plan* p, int a0, int a1, int slot) {
+void collect_timing(nlm_plan* p, int a0, int a1, int slot) {
   if (!p->timing) return;
   float ms = 0;
   if (hipEventSynchronize(p->ev[a1]) == hipSuccess && hipEventElapsedTime(&ms, p->ev[a0], p->ev[a1]) == hipSuccess)
@@ -536,6 +500,9 @@ extern "C" int nlm_pump_linear(nlm_plan* p, int n_pump, const double* envelope, 
     HIP_TRY(hipMemcpyAsync(p->strength.p, strength, sizeof(double) * nZP, hipMemcpyHostToDevice, p->stream));
   }
   if (int e = p->F0.ensure((size_t)p->nP * N)) return e;
+  p->pumpLinear = true;
+  p->hasStrength = strength != nullptr;
+  p->dzp = dzp;
   tic(p, 0);
   pick_fft(p->L)(p->env.p, p->F0.p, p->nP, p->tw, false, p->stream);
   for (int q = 0; q < p->nP; q++)
@@ -559,6 +526,7 @@ extern "C" int nlm_pump_chi3(nlm_plan* p, const double* envelope, const double* 
   HIP_TRY(hipMemcpyAsync(p->Dp.p, disp_pump, sizeof(double) * N, hipMemcpyHostToDevice, p->stream));
   if (int e = upload(p->dsp, disp_step_pump, N, p->stream)) return e;
   if (int e = upload(p->relI, rel_intensity, nZP, p->stream)) return e;
+  p->pumpLinear = false;
   tic(p, 0);
   pick_pump_chi3(p->L)(p->env.p, p->Dp.p, dzp, p->dsp.p, p->relI.p, nZP, p->pumpF[0].p, p->pumpT[0].p, p->tw,
                        p->stream);
@@ -576,6 +544,7 @@ extern "C" int nlm_pump_upload(nlm_plan* p, int q, const double* pump_time, cons
   if (pump_time) HIP_TRY(hipMemcpyAsync(p->pumpT[q].p, pump_time, cnt * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
   if (pump_freq) HIP_TRY(hipMemcpyAsync(p->pumpF[q].p, pump_freq, cnt * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
+  p->pumpLinear = false;
   return NLM_OK;
 }
 
@@ -590,7 +559,7 @@ extern "C" int nlm_pump_download(nlm_plan* p, int q, double* pump_time, double* 
   return NLM_OK;
 }
 
-static void fill_common(nlm_plan* p, KParams& kp) {
+void fill_common(nlm_plan* p, KParams& kp) {
   memset(&kp, 0, sizeof kp);
   kp.nZ = p->nZ;
   kp.tw = p->tw;
@@ -609,7 +578,7 @@ static void fill_common(nlm_plan* p, KParams& kp) {
   }
 }
 
-static int check_pumps_ready(nlm_plan* p) {
+int check_pumps_ready(nlm_plan* p) {
   for (int q = 0; q < p->nP; q++)
     if (!p->pumpT[q].p) return set_err(NLM_RUNTIME_ERROR, "pump grid has not been computed (run the pump simulation first)");
   return NLM_OK;
@@ -626,7 +595,7 @@ static void input_channels(int nS, int nCh, int inputMode, int* inPos) {
   }
 }
 
-static int launch_prop_kernel(nlm_plan* p, const KParams& kp) {
+int launch_prop_kernel(nlm_plan* p, const KParams& kp) {
   prop_fn fn = nlm_pick_prop(p->model, p->L);
   if (!fn) return set_err(NLM_INVALID_ARGUMENT, "unsupported model / size");
   int fpb = 1, T = p->N / 8;
@@ -793,6 +762,16 @@ extern "C" int nlm_greens_download(nlm_plan* p, double* C, double* S) {
   if (C) HIP_TRY(hipMemcpyAsync(C, p->C.p, total * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
   if (S) HIP_TRY(hipMemcpyAsync(S, p->S.p, total * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
+  return NLM_OK;
+}
+
+int nlm_fft_device(const cplx* in, cplx* out, int n, int count, bool inverse, hipStream_t s) {
+  int L;
+  if (int e = check_n(n, &L)) return e;
+  const cplx* tw = nullptr;
+  if (int e = get_twiddles(n, &tw)) return e;
+  pick_fft(L)(in, out, count, tw, inverse, s);
+  HIP_TRY(hipGetLastError());
   return NLM_OK;
 }
 

@@ -57,6 +57,15 @@ struct FftShape {
 
 __device__ __forceinline__ int lpad(int x) { return x + (x >> 3); }
 
+// Workgroup barrier ordering LDS only: unlike __syncthreads() it does not wait
+// for outstanding global loads, so loads issued before a transform (the next
+// z step's pump rows) stay in flight across its LDS exchanges.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // In-place R-point DFT of v[o], v[o+1], ..., v[o+R-1]; outputs in natural order.
 template <int R, bool INV>
 __device__ __forceinline__ void dft_small(cplx* v) {
@@ -134,7 +143,7 @@ __device__ __forceinline__ void fft_stages(cplx (&v)[8], cplx* __restrict__ buf,
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T, S = F::S;
   constexpr int R = F::radix(s), P = F::before(s), Mp = N / P, M = Mp / R, NB = 8 / R;
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int j = 0; j < NB; j++) {
     const int beta = t + T * j;
@@ -155,7 +164,7 @@ __device__ __forceinline__ void fft_stages(cplx (&v)[8], cplx* __restrict__ buf,
       for (int m = 0; m < 8; m++) v[m] = o[m];
     }
   } else {
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       const int beta = t + T * j;
@@ -181,7 +190,7 @@ __device__ __forceinline__ void fft_field(cplx (&v)[8], cplx* __restrict__ buf, 
   if constexpr (S > 1) {
 #pragma unroll
     for (int c = 1; c < 8; c++) v[c] = twmul<INV>(v[c], tw.w[0][c]);
-    __syncthreads();  // previous readers of buf are done
+    lds_barrier();  // previous readers of buf are done
 #pragma unroll
     for (int c = 0; c < 8; c++) buf[lpad(c * T + t)] = v[c];
     fft_stages<L, INV, 1>(v, buf, t, tw);

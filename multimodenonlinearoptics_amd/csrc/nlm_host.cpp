@@ -528,6 +528,48 @@ void Medium::greensOnDevice(bool inTimeDomain, bool runPump) {
   greensImpl(inTimeDomain, runPump, 1, false, {}, {}, false);
 }
 
+std::pair<Grid, Grid> Medium::greensForward(bool inTimeDomain, bool runPump, int ckptEvery) {
+  if (model_ != NLM_CHI2PDC) throw std::invalid_argument("the differentiable Green's function is implemented for Chi2PDC");
+  if (runPump) runPumpSimulation();
+  nlm_plan* p = plan();
+  for (int q = 0; q < nP_; q++)
+    if (pumpDevStale_[q]) uploadPump(q);
+  if (nP_ > 0 && !pumpComputed_) {
+    for (int q = 0; q < nP_; q++) uploadPump(q);
+    pumpComputed_ = true;
+  }
+  std::pair<Grid, Grid> res;
+  res.first.setZero(N_, N_);
+  res.second.setZero(N_, N_);
+  const int32_t m0 = 0;
+  check(nlm_greens_forward(p, inTimeDomain ? 1 : 0, 1, &m0, 1, &m0, nullptr, ckptEvery,
+                           reinterpret_cast<double*>(res.first.v.data()), reinterpret_cast<double*>(res.second.v.data())));
+  return res;
+}
+
+// Chain rule from the step coefficients to the per-step poling values for
+// Chi2PDC (solvers/Chi2PDC.cpp:35-46): c_prev = p[i-1] nl mis(i-1),
+// c_intm = (p[i-1] + p[i]) / 2 nl mis(i-1/2), c_curr = p[i] nl mis(i).
+std::pair<std::vector<cd>, std::vector<double>> Medium::greensAdjoint(const Grid& gC, const Grid& gS) {
+  if (model_ != NLM_CHI2PDC) throw std::invalid_argument("the differentiable Green's function is implemented for Chi2PDC");
+  if (gC.rows != N_ || gC.cols != N_ || gS.rows != N_ || gS.cols != N_)
+    throw std::invalid_argument("gradient shapes do not match the Green's function");
+  nlm_plan* p = plan();
+  std::vector<cd> gcoef((size_t)nZ_ * 3), genv(N_);
+  check(nlm_greens_adjoint(p, reinterpret_cast<const double*>(gC.v.data()), reinterpret_cast<const double*>(gS.v.data()),
+                           reinterpret_cast<double*>(gcoef.data()), nullptr, reinterpret_cast<double*>(genv.data())));
+  std::vector<double> gpol(nZ_, 0.0);
+  for (unsigned i = 1; i < (unsigned)nZ_; i++) {
+    const double x[3] = {(i - 1) * dz_, (i - .5) * dz_, i * dz_};
+    const cd d0 = nlStep_[0] * mismatch(0, x[0]), d1 = nlStep_[0] * mismatch(0, x[1]), d2 = nlStep_[0] * mismatch(0, x[2]);
+    const cd* g = gcoef.data() + (size_t)i * 3;
+    const double a0 = (std::conj(g[0]) * d0).real(), a1 = (std::conj(g[1]) * d1).real(), a2 = (std::conj(g[2]) * d2).real();
+    gpol[i - 1] += a0 + 0.5 * a1;
+    gpol[i] += 0.5 * a1 + a2;
+  }
+  return {genv, gpol};
+}
+
 void Medium::enableTiming(bool on) { check(nlm_plan_enable_timing(plan(), on ? 1 : 0)); }
 
 std::vector<double> Medium::lastTiming() {

@@ -95,6 +95,11 @@ class Medium {
   // Green's function with C and S left in device memory (no host copy); the
   // default mode selection of computeGreensFunction.  For timing the engine.
   void greensOnDevice(bool inTimeDomain, bool runPump);
+  // differentiable Green's function (single-mode Chi2PDC): forward with field
+  // checkpoints every ckptEvery steps (0: ceil(sqrt(nZ-1))), then the adjoint;
+  // returns (dL/d envelope [N] complex, dL/d poling [nZ] real) for dL/dC, dL/dS
+  std::pair<Grid, Grid> greensForward(bool inTimeDomain, bool runPump, int ckptEvery);
+  std::pair<std::vector<cd>, std::vector<double>> greensAdjoint(const Grid& gC, const Grid& gS);
   void enableTiming(bool on);
   std::vector<double> lastTiming();
   void sync();

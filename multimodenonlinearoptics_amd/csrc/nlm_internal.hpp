@@ -1,0 +1,74 @@
+// nlm_internal.hpp -- plan state and host helpers shared by the engine's
+// translation units (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nlm.h"
+#include "nlm_prop.hpp"
+
+int set_err(int kind, const std::string& msg);
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return set_err(NLM_DEVICE_ERROR, std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int ensure(size_t count) {
+    if (count <= n && p) return NLM_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) return NLM_OK;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) return set_err(NLM_DEVICE_ERROR, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    n = count;
+    return NLM_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct nlm_plan_s {
+  int model = 0, N = 0, L = 0, nZ = 0, nS = 0, nP = 0, NC = 0, NCC = 0;
+  int device = 0;
+  hipStream_t own = nullptr, stream = nullptr;
+  const cplx* tw = nullptr;
+  DevBuf<cplx> dstep, halfIn, halfOut, coef, cconst, raman, filt;
+  DevBuf<cplx> pumpT[2], pumpF[2];
+  DevBuf<cplx> gridF, gridT;
+  DevBuf<cplx> G, C, S, in, out, env, F0, dsp, relI;
+  DevBuf<double> Dp, strength;
+  int greensRows = 0, greensCols = 0;
+  bool timing = false;
+  hipEvent_t ev[6] = {};
+  double last_ms[3] = {0, 0, 0};
+  // Green's forward kept for the adjoint
+  bool pumpLinear = false, hasStrength = false;
+  double dzp = 0;
+  DevBuf<cplx> ckpt, seg, gG, gC, gS, gcoefPart, gcoef, gpumpPart, gpump, gF0Part, gF0, gtmp;
+  DevBuf<cplx> scale;
+  int ckptEvery = 0, nSeg = 0, gNIn = 0, gNOut = 0, gInTime = 0, gHasScale = 0;
+};
+
+void fill_common(nlm_plan_s* p, KParams& kp);
+int check_pumps_ready(nlm_plan_s* p);
+int launch_prop_kernel(nlm_plan_s* p, const KParams& kp);
+int get_twiddles(int n, const cplx** out);
+void tic(nlm_plan_s* p, int slot);
+void collect_timing(nlm_plan_s* p, int a0, int a1, int slot);
+void launch_greens_assemble(const cplx* G, cplx* C, cplx* S, int N, int nIn, int nOut, int nb, const cplx* scale,
+                            hipStream_t s);

@@ -47,6 +47,8 @@ struct KParams {
   const cplx* pump1;
   const cplx* raman;
   const cplx* filt;
+  cplx* ckpt;          // optional: time-domain field every ckptEvery steps, [nFields][nSeg][NS][N]
+  int ckptEvery, nSeg;
 };
 
 template <int L>
@@ -202,23 +204,27 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop(KParams kp) {
       fft_field<L, true>(a[q], buf, t, tw);
       scale_inv<L>(a[q]);
       if (kp.kind == KIND_GRID) store_row<L>(kp.gridT + (size_t)q * kp.nZ * N, a[q], t, active);
+      if (kp.ckpt) store_row<L>(kp.ckpt + ((size_t)field * kp.nSeg * NS + q) * N, a[q], t, active);
     }
   }
 
   constexpr int NPP = NP > 0 ? NP : 1;
+  // pump rows 2i-2 (pp), 2i-1 (pi), 2i (pc) of step i; the next step's rows are
+  // loaded before this step's transforms so their latency hides behind them
   cplx pp[NPP][8], pi[NPP][8], pc[NPP][8];
   const cplx* pumps[2] = {kp.pump0, kp.pump1};
 #pragma unroll
-  for (int r = 0; r < NP; r++) load_pump<L>(pp[r], pumps[r], t);
+  for (int r = 0; r < NP; r++) {
+    load_pump<L>(pp[r], pumps[r], t);
+    if (kp.nZ > 1) {
+      load_pump<L>(pi[r], pumps[r] + (size_t)N, t);
+      load_pump<L>(pc[r], pumps[r] + (size_t)2 * N, t);
+    }
+  }
 
   for (int i = 1; i < kp.nZ; i++) {
     // --- RK4 nonlinear step (DiffEq + update, src/_NonlinearMedium.hpp:207-210)
     const cplx* c = kp.coef + (size_t)i * 3 * NC;
-#pragma unroll
-    for (int r = 0; r < NP; r++) {
-      load_pump<L>(pi[r], pumps[r] + (size_t)(2 * i - 1) * N, t);
-      load_pump<L>(pc[r], pumps[r] + (size_t)(2 * i) * N, t);
-    }
 #pragma unroll
     for (int m = 0; m < 8; m++) {
       cplx x[NS], y[NS], k1[NS], k2[NS], k3[NS], k4[NS], P0[NPP], P1[NPP], P2[NPP];
@@ -244,9 +250,14 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop(KParams kp) {
       for (int q = 0; q < NS; q++) a[q][m] = x[q] + (((k1[q] + 2.0 * k2[q]) + 2.0 * k3[q]) + k4[q]) * (1. / 6.);
     }
 #pragma unroll
-    for (int r = 0; r < NP; r++)
+    for (int r = 0; r < NP; r++) {
 #pragma unroll
       for (int m = 0; m < 8; m++) pp[r][m] = pc[r][m];
+      if (i + 1 < kp.nZ) {
+        load_pump<L>(pi[r], pumps[r] + (size_t)(2 * i + 1) * N, t);
+        load_pump<L>(pc[r], pumps[r] + (size_t)(2 * i + 2) * N, t);
+      }
+    }
 
     // --- dispersion step (src/_NonlinearMedium.hpp:212-215)
 #pragma unroll
@@ -259,6 +270,8 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop(KParams kp) {
         fft_field<L, true>(a[q], buf, t, tw);
         scale_inv<L>(a[q]);
         if (kp.kind == KIND_GRID) store_row<L>(kp.gridT + ((size_t)q * kp.nZ + i) * N, a[q], t, active);
+        if (kp.ckpt && i % kp.ckptEvery == 0)
+          store_row<L>(kp.ckpt + (((size_t)field * kp.nSeg + i / kp.ckptEvery) * NS + q) * N, a[q], t, active);
       }
     }
   }

@@ -232,6 +232,33 @@ PYBIND11_MODULE(nonlinearmedium, m) {
            },
            "Compute the Green's function and keep C and S in device memory (extension, for timing).",
            "inTimeDomain"_a = false, "runPump"_a = true);
+  base.def("greensForward",
+           [](Medium& md, bool inTimeDomain, bool runPump, int ckptEvery) {
+             std::pair<Grid, Grid> r;
+             {
+               py::gil_scoped_release nogil;
+               r = md.greensForward(inTimeDomain, runPump, ckptEvery);
+             }
+             return py::make_tuple(owned(std::move(r.first)), owned(std::move(r.second)));
+           },
+           "Green's function that keeps field checkpoints for greensAdjoint (extension; Chi2PDC).",
+           "inTimeDomain"_a = false, "runPump"_a = true, "ckptEvery"_a = 0);
+  base.def("greensAdjoint",
+           [](Medium& md, const carr& gC, const carr& gS) {
+             Grid a = to_grid(gC), b = to_grid(gS);
+             std::pair<std::vector<cd>, std::vector<double>> r;
+             {
+               py::gil_scoped_release nogil;
+               r = md.greensAdjoint(a, b);
+             }
+             py::array_t<std::complex<double>> ge(r.first.size());
+             std::copy(r.first.begin(), r.first.end(), ge.mutable_data());
+             py::array_t<double> gp(r.second.size());
+             std::copy(r.second.begin(), r.second.end(), gp.mutable_data());
+             return py::make_tuple(ge, gp);
+           },
+           "Gradients (dL/d envelope, dL/d poling profile) of the last greensForward for dL/dC, dL/dS "
+           "(convention dL/dRe + i dL/dIm) (extension; Chi2PDC).", "gC"_a, "gS"_a);
   base.def("enableTiming", &Medium::enableTiming, "Record HIP-event kernel times of each run (extension).", "on"_a);
   base.def("lastTiming", &Medium::lastTiming,
            "Kernel times of the last run in ms: [pump, propagation, assembly] (extension).");

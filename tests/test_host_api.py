@@ -1,0 +1,201 @@
+"""CPU checks of the drop-in boundary: the C-ABI library and the pybind11 module.
+
+No GPU is used here: the library is loaded and its exports checked against
+include/nlm.h; the Python classes are constructed (construction is host-only)
+and their tables, signatures and errors compared with the reference's
+contract and with the oracle.
+"""
+import inspect
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cases import ALL, make, small_case
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(REPO, "include", "nlm.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(nlm_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    from multimodenonlinearoptics_amd import capi
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(capi.SYMBOLS)
+
+
+def test_library_reports_errors_without_compute():
+    from multimodenonlinearoptics_amd import capi
+    lib = capi.lib()
+    assert lib.nlm_version().startswith(b"nlm-hip")
+    import ctypes
+    nc = ctypes.c_int32()
+    assert lib.nlm_model_ncoef(1, ctypes.byref(nc), None, None, None) == 0 and nc.value == 1
+    assert lib.nlm_model_ncoef(99, None, None, None, None) == 1
+    assert b"unknown model" in lib.nlm_last_error()
+
+
+def test_module_surface(nlm):
+    for name in ALL + ["Cascade", "_NonlinearMedium", "_FullyNonlinearMedium", "PulseType", "IntensityProfile"]:
+        assert hasattr(nlm, name), name
+    assert issubclass(nlm.Chi2PDC, nlm._NonlinearMedium)
+    assert issubclass(nlm.Chi2SHG, nlm._FullyNonlinearMedium)
+    assert issubclass(nlm._FullyNonlinearMedium, nlm._NonlinearMedium)
+    assert issubclass(nlm.Cascade, nlm._NonlinearMedium)
+    assert int(nlm.Sech) == 1 and int(nlm.GaussianApodization) == 2
+
+
+# argument names and defaults from the reference's NLMMODULE blocks
+SIGNATURES = {
+    "Chi3": "relativeLength, nlLength, beta2, customPump=..., pulseType=..., beta3=0, rayleighLength=inf, tMax=10, "
+            "tPrecision=512, zPrecision=100, intensityProfile=..., chirp=0",
+    "Chi2PDC": "relativeLength, nlLength, beta2, beta2s, customPump=..., pulseType=..., beta1=0, beta1s=0, beta3=0, "
+               "beta3s=0, diffBeta0=0, rayleighLength=inf, tMax=10, tPrecision=512, zPrecision=100, "
+               "intensityProfile=..., chirp=0, delay=0, poling=...",
+    "Chi2SHG": "relativeLength, nlLengthH, nlLengthP, beta2h, beta2p, beta1h=0, beta1p=0, beta3h=0, beta3p=0, "
+               "diffBeta0=0, rayleighLength=inf, tMax=10, tPrecision=512, zPrecision=100, intensityProfile=..., "
+               "poling=...",
+    "Chi3GNLSE": "relativeLength, nlLength, selfSteepLength, fr, fb, tau1, tau2, tau3, beta2, beta3=0, "
+                 "rayleighLength=inf, tMax=10, tPrecision=512, zPrecision=100, intensityProfile=...",
+}
+
+
+def _doc_args(cls):
+    doc = cls.__init__.__doc__
+    m = re.search(r"__init__\(self: [^,]+, (.*)\) -> None", doc)
+    return m.group(1)
+
+
+@pytest.mark.parametrize("name", list(SIGNATURES))
+def test_constructor_signatures(nlm, name):
+    sig = _doc_args(getattr(nlm, name))
+    got = []
+    parts, depth, cur = [], 0, ""
+    for ch in sig:
+        if ch in "([{":
+            depth += 1
+        elif ch in ")]}":
+            depth -= 1
+        if ch == "," and depth == 0:
+            parts.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    parts.append(cur)
+    for part in parts:
+        part = part.strip()
+        if part in ("*",) or part.startswith("nZSteps") or part.startswith("polingProfile"):
+            continue
+        nm = part.split(":")[0].strip()
+        default = part.split("=")[1].strip() if "=" in part else None
+        got.append((nm, default))
+    exp = []
+    for part in SIGNATURES[name].split(", "):
+        if "=" in part:
+            k, v = part.split("=")
+            exp.append((k, v))
+        else:
+            exp.append((part, None))
+    assert [g[0] for g in got] == [e[0] for e in exp]
+    for (gk, gv), (ek, ev) in zip(got, exp):
+        if ev not in (None, "..."):
+            assert float(gv) == float(ev), (gk, gv, ev)
+
+
+@pytest.mark.parametrize("name", ALL)
+def test_host_tables_equal_oracle(nlm, oracle, name):
+    _, kw = small_case(name, nlm, N=64, nZ=33, seed=2)
+    _, kwo = small_case(name, oracle, N=64, nZ=33, seed=2)
+    dev = make(nlm, name, kw)
+    ref = make(oracle, name, kwo)
+    assert np.array_equal(np.asarray(dev.omega), ref.omega)
+    assert np.array_equal(np.asarray(dev.tau), ref.tau)
+    assert np.array_equal(np.asarray(dev.poling), ref.poling)
+    assert dev.nZSteps == ref.nZSteps == 33
+
+
+def test_derived_step_count_matches_oracle(nlm, oracle):
+    a = nlm.Chi3(relativeLength=math.pi / 2, nlLength=1, beta2=-1, zPrecision=637)
+    b = oracle.Chi3(relativeLength=math.pi / 2, nlLength=1, beta2=-1, zPrecision=637)
+    assert a.nZSteps == b.nZSteps == 1000
+    a = nlm.Chi2PDC(relativeLength=3.0, nlLength=0.25, beta2=1, beta2s=-2, rayleighLength=0.7, zPrecision=50)
+    b = oracle.Chi2PDC(relativeLength=3.0, nlLength=0.25, beta2=1, beta2s=-2, rayleighLength=0.7, zPrecision=50)
+    assert a.nZSteps == b.nZSteps
+
+
+@pytest.mark.parametrize("kw,exc,msg", [
+    (dict(tPrecision=511), ValueError, "Invalid number of Frequencies"),
+    (dict(tMax=-1.0), ValueError, "Negative time span"),
+    (dict(relativeLength=-1.0), ValueError, "Non-positive length scale"),
+    (dict(zPrecision=0), ValueError, "Zero steps"),
+    (dict(poling=[1.0, -1.0]), ValueError, "Poling contains invalid domain length"),
+    (dict(customPump=np.ones(7)), ValueError, "Custom pump array length does not match"),
+])
+def test_constructor_errors_match_reference(nlm, oracle, kw, exc, msg):
+    base = dict(relativeLength=1.0, nlLength=1.0, beta2=0.5, beta2s=0.5, tPrecision=64)
+    base.update(kw)
+    with pytest.raises(exc, match=msg):
+        nlm.Chi2PDC(**base)
+    with pytest.raises(exc, match=msg):
+        oracle.Chi2PDC(**base)
+
+
+def test_method_errors_match_reference(nlm):
+    m = nlm.Chi2PDC(1.0, 1.0, 0.5, 0.5, tPrecision=64, nZSteps=10)
+    with pytest.raises(ValueError, match="inputProf array size does not match"):
+        m.runSignalSimulation(np.ones(63))
+    with pytest.raises(ValueError, match="inputModes does not match any mode"):
+        m.runSignalSimulation(np.ones(64), True, 1)
+    with pytest.raises(ValueError, match="Too many threads requested!"):
+        m.computeGreensFunction(nThreads=65)
+    with pytest.raises(ValueError, match="Signals not of correct length!"):
+        m.batchSignalSimulation(np.ones((2, 65)))
+    with pytest.raises(ValueError, match="Too many threads requested!"):
+        m.batchSignalSimulation(np.ones((2, 64)), nThreads=3)
+    with pytest.raises(ValueError, match="List of requested inputs indices longer"):
+        m.computeGreensFunction(useInput=[1, 1])
+    with pytest.raises(ValueError, match="Invalid pump index"):
+        m.setPump(nlm.Gaussian, 0, 0, 1)
+    s = nlm.Chi2SHG(1.0, 1.0, 1.0, 0.2, 0.3, tPrecision=32, nZSteps=10)
+    for call in (s.runPumpSimulation, s.computeGreensFunction, lambda: s.setPump(nlm.Gaussian)):
+        with pytest.raises(RuntimeError, match="Object does not have this method."):
+            call()
+
+
+def test_chi2sfgii_keeps_reference_slot_order(nlm):
+    # the reference binds "intensityProfile" to the uint zPrecision slot and
+    # "zPrecision" to the IntensityProfile slot (solvers/Chi2SFGII.cpp:97-106)
+    args = (1.0, 1.0, 1.0, 0.1, 0.2, 0.3, 0.4, 0.5)
+    with pytest.raises(TypeError):
+        nlm.Chi2SFGII(*args)  # default zPrecision=100 is not an IntensityProfile
+    m = nlm.Chi2SFGII(*args, intensityProfile=40, zPrecision=nlm.Constant, tPrecision=32)
+    assert m.nZSteps == int(40 * 1.0 / min(1.0, 1 / 0.5))
+
+
+def test_device_call_without_gpu_fails_loudly(nlm):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    m = nlm.Chi2PDC(1.0, 1.0, 0.5, 0.5, tPrecision=64, nZSteps=10)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        m.computeGreensFunction()
+
+
+def test_grid_views_are_read_only_and_shaped(nlm):
+    m = nlm.Chi2SFG(1.0, 1.0, 1.0, 0.1, 0.2, 0.3, tPrecision=32, nZSteps=12)
+    assert m.signalFreq.shape == (12, 32) and m.pumpTime.shape == (23, 32)
+    assert m.signalTimes(1).shape == (12, 32)
+    assert not m.signalFreq.flags.writeable
+    with pytest.raises(IndexError):
+        m.signalTimes(2)
