@@ -53,7 +53,7 @@ static padj_fn pick_padj(int L) { NLM_L_SWITCH2(launch_padj, L) }
 
 static int adj_threads(int N) {
   const int T = N / 8;
-  return T >= 512 ? T : 512;
+  return T >= 256 ? T : 256;
 }
 
 extern "C" int nlm_greens_forward(nlm_plan* p, int in_time, int n_in, const int32_t* inputs, int n_out,

@@ -67,7 +67,9 @@ struct Adj<MChi2PDC> {
 template <int L>
 struct AdjLaunch {
   static constexpr int T = FftShape<L>::T;
-  static constexpr int FA = T >= 512 ? 1 : 512 / T;  // fields per workgroup
+  // 256 threads (one wave per SIMD) leave each wave the full register file:
+  // the adjoint holds the field, its gradient and three pump-row gradients
+  static constexpr int FA = T >= 256 ? 1 : 256 / T;  // fields per workgroup
   static constexpr int THREADS = FA * T;
   static constexpr int WAVES = (THREADS + 63) / 64;
 };
