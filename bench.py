@@ -46,6 +46,81 @@ def flops_per_field_step(n):
     return 10 * n * math.log2(n) + 80 * n
 
 
+def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6):
+    """Differentiable Chi2PDC (N=1024, nZ=2000): loss ||S S^T - 1.5 I||_F^2 of the
+    Green's function S, gradient w.r.t. the pump spectral phase (1024 reals) and the
+    per-step poling (2000 reals) by the adjoint kernel with ~sqrt(nZ) checkpoints.
+
+    Reports the backward time over one forward Green's solve (no checkpoints), both
+    as wall time of the calls and as kernel time, and the worst relative error of
+    the directional derivative against central finite differences of the device
+    forward along n_dirs seeded unit directions."""
+    import numpy as np
+    from cases import SEED0, config4
+    m, spec0, phi, pol = config4(nlm)
+    N, nZ = spec0.size, pol.size
+    m.enableTiming(True)
+
+    def env_of(ph):
+        return np.fft.ifft(spec0 * np.exp(1j * ph))
+
+    def loss_and_gS(S):
+        M = S @ S.T - 1.5 * np.eye(N)
+        return float(np.sum(np.abs(M) ** 2)), 4.0 * (M @ np.conj(S))
+
+    def loss_at(ph, pl):
+        m.setPump(env_of(ph))
+        m.setPolingProfile(pl)
+        return loss_and_gS(m.computeGreensFunction()[1])[0]
+
+    m.greensOnDevice()
+    m.sync()
+    t_fwd, k_fwd = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        m.greensOnDevice()
+        m.sync()
+        t_fwd.append(time.perf_counter() - t0)
+        k_fwd.append(m.lastTiming()[1])
+    t_ck, t_bwd, k_bwd = [], [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        C, S = m.greensForward(False, True, 0)
+        t_ck.append(time.perf_counter() - t0)
+        L0, gS = loss_and_gS(S)
+        t0 = time.perf_counter()
+        g_env, g_pol = m.greensAdjoint(np.zeros_like(C), gS)
+        t_bwd.append(time.perf_counter() - t0)
+        tm = m.lastTiming()
+        k_bwd.append(tm[1] + tm[2])
+    # chain rule envelope -> spectral phase: env = IFFT(X), X = spec0 exp(i phi)
+    X = spec0 * np.exp(1j * phi)
+    gX = np.fft.fft(g_env) / N
+    g_phi = np.real(np.conj(gX) * 1j * X)
+    r = np.random.default_rng(SEED0 + 5)
+    worst = 0.0
+    for _ in range(n_dirs):
+        d = r.standard_normal(N + nZ)
+        d /= np.linalg.norm(d)
+        dphi, dpol = d[:N], d[N:]
+        fd = (loss_at(phi + fd_eps * dphi, pol + fd_eps * dpol) - loss_at(phi - fd_eps * dphi, pol - fd_eps * dpol)) / (
+            2 * fd_eps)
+        ad = float(np.dot(g_phi, dphi) + np.dot(g_pol, dpol))
+        worst = max(worst, abs(ad - fd) / max(abs(fd), 1e-30))
+    m.setPump(env_of(phi))
+    m.setPolingProfile(pol)
+    return {
+        "workload": "Chi2PDC N=1024 nZ=2000, loss ||S S^T - 1.5 I||_F^2, grad w.r.t. pump spectral phase "
+                    "(1024) and per-step poling (2000)",
+        "loss": L0,
+        "forward_s": min(t_fwd), "forward_ckpt_s": min(t_ck), "backward_s": min(t_bwd),
+        "forward_kernel_ms": min(k_fwd), "backward_kernel_ms": min(k_bwd),
+        "grad_vs_forward": min(t_bwd) / min(t_fwd),
+        "grad_vs_forward_kernel": min(k_bwd) / min(k_fwd),
+        "fd_directions": n_dirs, "fd_eps": fd_eps, "fd_max_rel_err": worst,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +128,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
+    ap.add_argument("--no-grad", action="store_true", help="skip the adjoint (gradient) measurement")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (default: min(16, cores))")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="only run a device-to-device copy of this many MiB (calibrates HBM byte counters)")
@@ -140,10 +216,16 @@ def main():
                   "s_per_solve": plain_solve_s},
         "batched_vs_plain": value / plain_value if plain_value else None,
         "grad_vs_forward": None,
+        "grad": None,
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
                      "kernel": "k_prop<MChi2PDC, 9>", "flops_per_field_step": fl},
     }
+
+    if not args.no_grad:
+        g = grad_benchmark(nlm)
+        rec["grad"] = g
+        rec["grad_vs_forward"] = g["grad_vs_forward"]
 
     if not args.no_cpu:
         from oracle import oracle as O

@@ -141,6 +141,12 @@ int nlm_batch_dev(nlm_plan* plan, const double* inputs_dev, int rows, int cols, 
 
 /* Per-launch device timing of the last call (ms), recorded with HIP events on
  * the plan's stream when enabled: [0] pump, [1] propagation, [2] assembly. */
+/* Single-signal-mode undepleted media (Chi3, Chi2PDC): propagate with the
+ * per-step RK4 propagator table (enable = 1, the default) or evaluate the
+ * DiffEq in every field (enable = 0).  Both compute the RK4 step of
+ * src/_NonlinearMedium.hpp:207-210; they differ by rounding only.  (Extension:
+ * no counterpart in the reference.) */
+int nlm_plan_set_step_table(nlm_plan* plan, int enable);
 int nlm_plan_enable_timing(nlm_plan* plan, int enable);
 int nlm_plan_last_timing(nlm_plan* plan, double ms[3]);
 

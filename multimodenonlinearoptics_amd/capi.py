@@ -14,7 +14,7 @@ SYMBOLS = [
     "nlm_model_ncoef", "nlm_plan_create", "nlm_plan_destroy", "nlm_plan_set_stream", "nlm_plan_stream",
     "nlm_plan_set_coef", "nlm_pump_linear", "nlm_pump_chi3", "nlm_pump_upload", "nlm_pump_download",
     "nlm_propagate_grid", "nlm_grid_download", "nlm_batch", "nlm_greens", "nlm_greens_download",
-    "nlm_batch_dev", "nlm_plan_enable_timing", "nlm_plan_last_timing", "nlm_greens_compose", "nlm_fft",
+    "nlm_batch_dev", "nlm_plan_set_step_table", "nlm_plan_enable_timing", "nlm_plan_last_timing", "nlm_greens_compose", "nlm_fft",
     "nlm_plan_sync", "nlm_last_error", "nlm_version", "nlm_greens_forward", "nlm_greens_adjoint",
 ]
 
@@ -45,6 +45,7 @@ def lib():
         L.nlm_plan_stream.restype = vp
         L.nlm_plan_set_stream.argtypes = [vp, vp]
         L.nlm_plan_enable_timing.argtypes = [vp, C.c_int]
+        L.nlm_plan_set_step_table.argtypes = [vp, C.c_int]
         L.nlm_plan_last_timing.argtypes = [vp, dp]
         L.nlm_plan_sync.argtypes = [vp]
         L.nlm_pump_linear.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp]

@@ -18,18 +18,10 @@ __global__ void k_env_grad(const cplx* __restrict__ ifft, const cplx* __restrict
 int nlm_fft_device(const cplx* in, cplx* out, int n, int count, bool inverse, hipStream_t s);
 
 typedef void (*adj_fn)(const AParams&, int grid, hipStream_t);
-typedef void (*padj_fn)(const cplx*, const double*, double, const double*, int, cplx*, const cplx*, int, hipStream_t);
 
 template <int L>
 static void launch_adj(const AParams& ap, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_adjoint<MChi2PDC, L>), dim3(grid), dim3(AdjLaunch<L>::THREADS), 0, s, ap);
-}
-template <int L>
-static void launch_padj(const cplx* gT, const double* Dp, double dzp, const double* st, int nZP, cplx* part,
-                        const cplx* tw, int groups, hipStream_t s) {
-  const int grid = groups / Launch<L>::FPB;
-  hipLaunchKernelGGL((k_pump_linear_adjoint<L>), dim3(grid), dim3(Launch<L>::THREADS), 0, s, gT, Dp, dzp, st, nZP,
-                     part, tw);
+  hipLaunchKernelGGL((k_adjoint_pdc<L>), dim3(grid), dim3(AdjLaunch<L>::THREADS), 0, s, ap);
 }
 
 #define NLM_L_SWITCH2(FN, L) \
@@ -49,7 +41,6 @@ static void launch_padj(const cplx* gT, const double* Dp, double dzp, const doub
   return nullptr;
 
 static adj_fn pick_adj(int L) { NLM_L_SWITCH2(launch_adj, L) }
-static padj_fn pick_padj(int L) { NLM_L_SWITCH2(launch_padj, L) }
 
 static int adj_threads(int N) {
   const int T = N / 8;
@@ -114,10 +105,11 @@ extern "C" int nlm_greens_forward(nlm_plan* p, int in_time, int n_in, const int3
 
 extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* gS, double* g_coef,
                                   double* g_pump_time, double* g_envelope) {
-  double* g_pump_spectrum = g_envelope;
   if (!p || !gC || !gS || !g_coef) return set_err(NLM_INVALID_ARGUMENT, "null argument");
   if (!p->nSeg || !p->ckpt.p) return set_err(NLM_RUNTIME_ERROR, "run nlm_greens_forward before the adjoint");
-  const int N = p->N, L = p->L, nZ = p->nZ, nZP = 2 * nZ - 1, NC = p->NC;
+  if (g_envelope && !p->pumpLinear)
+    return set_err(NLM_INVALID_ARGUMENT, "envelope gradient needs the linear pump simulation");
+  const int N = p->N, L = p->L, nZ = p->nZ, nZP = 2 * nZ - 1;
   const size_t nF = 2 * (size_t)N, W = N, total = W * N;
   const int TH = adj_threads(N);
   const int FA = TH / (N / 8);
@@ -125,18 +117,25 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
   const int nWaves = nWG * (TH / 64);
   int e;
   if ((e = p->gC.ensure(total)) || (e = p->gS.ensure(total)) || (e = p->gG.ensure(nF * W)) ||
-      (e = p->seg.ensure(nF * p->ckptEvery * N)) || (e = p->gcoefPart.ensure((size_t)nWaves * nZ * 3 * NC)) ||
-      (e = p->gcoef.ensure((size_t)nZ * 3 * NC)))
+      (e = p->seg.ensure(nF * p->ckptEvery * N)) || (e = p->gcoefPart.ensure((size_t)nWaves * nZ * 3)) ||
+      (e = p->gcoef.ensure((size_t)nZ * 3)) || (e = p->stepTab.ensure((size_t)nZ * N * 2)))
     return e;
-  const bool wantPump = g_pump_time || g_pump_spectrum;
-  if (wantPump) {
+  if (g_pump_time) {
     if ((e = p->gpumpPart.ensure((size_t)nWG * nZP * N)) || (e = p->gpump.ensure((size_t)nZP * N))) return e;
+  }
+  if (g_envelope) {
+    if ((e = p->gF0Part.ensure((size_t)nWG * 3 * N)) || (e = p->gF0.ensure(2 * (size_t)N)) ||
+        (e = p->gtmp.ensure(4 * (size_t)N)))
+      return e;
   }
   HIP_TRY(hipMemcpyAsync(p->gC.p, gC, total * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
   HIP_TRY(hipMemcpyAsync(p->gS.p, gS, total * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
-  HIP_TRY(hipMemsetAsync(p->gcoefPart.p, 0, (size_t)nWaves * nZ * 3 * NC * sizeof(cplx), p->stream));
+  HIP_TRY(hipMemsetAsync(p->gcoefPart.p, 0, (size_t)nWaves * nZ * 3 * sizeof(cplx), p->stream));
   hipLaunchKernelGGL(k_greens_assemble_adjoint, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, p->stream,
                      p->gC.p, p->gS.p, p->gG.p, N, 1, 1, 1, (const cplx*)nullptr);
+  // the recomputation between checkpoints uses the step propagators
+  nlm_launch_step_table(NLM_CHI2PDC, p->coef.p, p->cconst.p, p->pumpT[0].p, N, nZ, p->stepTab.p, p->stream);
+  HIP_TRY(hipGetLastError());
   AParams ap;
   memset(&ap, 0, sizeof ap);
   ap.nZ = nZ;
@@ -153,42 +152,49 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
   ap.gout = p->gG.p;
   ap.gin = nullptr;
   ap.gcoef = p->gcoefPart.p;
-  ap.gpump = wantPump ? p->gpumpPart.p : nullptr;
+  ap.gpump = g_pump_time ? p->gpumpPart.p : nullptr;
+  if (g_envelope) {
+    ap.genvF = p->gF0Part.p;                   // [nWG][2][N]
+    ap.genvT = p->gF0Part.p + (size_t)nWG * 2 * N;  // [nWG][N]
+    ap.Dp = p->Dp.p;
+    ap.ph1 = p->gtmp.p + 2 * N;
+    ap.ph2 = p->gtmp.p + 3 * N;
+    ap.dzp = p->dzp;
+    ap.strength = p->hasStrength ? p->strength.p : nullptr;
+    hipLaunchKernelGGL(k_pump_phase_steps, dim3((N + 255) / 256), dim3(256), 0, p->stream, p->Dp.p, p->dzp, N,
+                       p->gtmp.p + 2 * N, p->gtmp.p + 3 * N);
+  }
   ap.tw = p->tw;
   ap.dstep = p->dstep.p;
   ap.halfIn = p->halfIn.p;
   ap.halfOut = p->halfOut.p;
   ap.coef = p->coef.p;
-  ap.cconst = p->cconst.p;
   ap.pump0 = p->pumpT[0].p;
+  ap.tab = p->stepTab.p;
   tic(p, 2);
   pick_adj(L)(ap, nWG, p->stream);
   HIP_TRY(hipGetLastError());
   tic(p, 3);
-  const long nc = (long)nZ * 3 * NC;
+  const long nc = (long)nZ * 3;
   hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, p->stream, p->gcoefPart.p, nc,
                      nWaves, p->gcoef.p);
-  if (wantPump) {
+  if (g_pump_time) {
     const long np = (long)nZP * N;
     hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, p->stream, p->gpumpPart.p,
                        np, nWG, p->gpump.p);
   }
   HIP_TRY(hipGetLastError());
-  if (g_pump_spectrum) {
-    if (!p->pumpLinear) return set_err(NLM_INVALID_ARGUMENT, "envelope gradient needs the linear pump simulation");
-    int fpb = N / 8 >= 64 ? 1 : 64 / (N / 8);
-    int groups = 256 * fpb;
-    if ((e = p->gF0Part.ensure((size_t)groups * N)) || (e = p->gF0.ensure(N))) return e;
-    pick_padj(L)(p->gpump.p, p->Dp.p, p->dzp, p->hasStrength ? p->strength.p : nullptr, nZP, p->gF0Part.p, p->tw,
-                 groups, p->stream);
-    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, p->stream, p->gF0Part.p,
-                       (long)N, groups, p->gF0.p);
+  if (g_envelope) {
+    // g_F0 = sum of the workgroup partials; g_env = N IFFT(g_F0) + s_0 g_row0
+    hipLaunchKernelGGL(k_sum_partials, dim3((N + 255) / 256), dim3(256), 0, p->stream, ap.genvF, (long)N, 2 * nWG,
+                       p->gF0.p);
+    hipLaunchKernelGGL(k_sum_partials, dim3((N + 255) / 256), dim3(256), 0, p->stream, ap.genvT, (long)N, nWG,
+                       p->gF0.p + N);
     HIP_TRY(hipGetLastError());
-    if ((e = p->gtmp.ensure(2 * (size_t)N))) return e;
     if ((e = nlm_fft_device(p->gF0.p, p->gtmp.p, N, 1, true, p->stream))) return e;
     double s0 = 1.0;
     if (p->hasStrength) HIP_TRY(hipMemcpy(&s0, p->strength.p, sizeof(double), hipMemcpyDeviceToHost));
-    hipLaunchKernelGGL(k_env_grad, dim3((N + 255) / 256), dim3(256), 0, p->stream, p->gtmp.p, p->gpump.p, s0, N,
+    hipLaunchKernelGGL(k_env_grad, dim3((N + 255) / 256), dim3(256), 0, p->stream, p->gtmp.p, p->gF0.p + N, s0, N,
                        p->gtmp.p + N);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(g_envelope, p->gtmp.p + N, N * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));

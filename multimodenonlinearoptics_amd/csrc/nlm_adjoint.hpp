@@ -7,16 +7,19 @@
 // (b = F a) has g_a = N * IFFT(g_b) and the scaled inverse DFT has
 // g_a = FFT(g_b) / N.
 //
-// The forward Green's launch (k_prop with kp.ckpt set) stores the time-domain
-// field every K steps.  For each segment, in reverse order, k_adjoint
-// recomputes the segment's states from its checkpoint (the same code path as
-// the forward pass, so the states are bit-identical), then sweeps the segment
+// The forward Green's launch (k_prop_tab with kp.ckpt set) stores the
+// time-domain field every K steps.  For each segment, in reverse order,
+// k_adjoint_pdc recomputes the segment's states from its checkpoint with the
+// same step propagators as the forward pass, then sweeps the segment
 // backwards: adjoint of the dispersion step (forward FFT, conj(exp(i dz D)),
-// inverse FFT), then the transposed RK4 Jacobian of the model's DiffEq.  The
-// per-step coefficient gradients are reduced within each wave (deterministic
-// shuffles) and written as per-wave partials; the pump-row gradients are summed
-// over the workgroup's fields in LDS and written as per-workgroup partials;
-// both are summed by separate passes in a fixed order (no atomics).
+// inverse FFT), then the transposed RK4 Jacobian of the DiffEq.  The per-step
+// coefficient gradients are reduced within each wave (deterministic shuffles)
+// and written as per-wave partials.  The pump enters through its linear
+// propagation P_r = s_r IFFT(FFT(envelope) exp(i r dzp D_p)), so the gradient
+// of each pump row, summed over the workgroup's fields in LDS, is transformed
+// and folded into a per-workgroup accumulator of dL/dFFT(envelope) right
+// away; nothing of size nZ x N is written per workgroup.  All partials are
+// summed by separate passes in a fixed order (no atomics).
 //
 // Forward map replaced: signalSimulationTemplate (src/_NonlinearMedium.hpp:160-239)
 // with Chi2PDC::DiffEq (solvers/Chi2PDC.cpp:27-47); assembly of
@@ -29,39 +32,26 @@ struct AParams {
   int nZ, nFields, K, nSeg;
   int inTime, outTime, nOut;
   int outPos[4];
-  const cplx* ckpt;    // [nFields][nSeg][NS][N]
-  cplx* seg;           // [nFields][K][NS][N]
+  const cplx* ckpt;    // [nFields][nSeg][N] scaled time-domain states a_{sK}
+  cplx* seg;           // [nFields][K][N] recomputed states of the current segment
   const cplx* gout;    // [nFields][nOut*N], domain = outTime
-  cplx* gin;           // [nFields][NS*N] gradient w.r.t. the input (input domain) or nullptr
-  cplx* gcoef;         // [nWaves][nZ][3][NC]
-  cplx* gpump;         // [nWG][nZP][N] or nullptr
+  cplx* gin;           // [nFields][N] gradient w.r.t. the input (input domain) or nullptr
+  cplx* gcoef;         // [nWaves][nZ][3] per-wave partials of the step coefficients
+  cplx* gpump;         // [nWG][nZP][N] per-workgroup pump-row partials, or nullptr
+  cplx* genvF;         // [nWG][2][N] per-workgroup partials of dL/dFFT(envelope), or nullptr
+  cplx* genvT;         // [nWG][N] per-workgroup partials of dL/d(pump row 0) (with genvF)
+  const double* Dp;    // [N] pump dispersion D_p(omega)        (genvF)
+  const cplx* ph1;     // [N] exp(i dzp D_p), ph2 = exp(2 i dzp D_p) (genvF)
+  const cplx* ph2;
+  const double* strength;  // [2nZ-1] beam-profile factor per pump row, or nullptr
+  double dzp;
   const cplx* tw;
   const cplx* dstep;
   const cplx* halfIn;
   const cplx* halfOut;
-  const cplx* coef;
-  const cplx* cconst;
-  const cplx* pump0;
-};
-
-// Models with an adjoint: fadj<ST>(c, cc, y, P, gk, gy, gm) returns the
-// gradient w.r.t. the stage input y (gy) and w.r.t. m = c*P (gm) of
-// k = f<ST>(c, y, P) for an upstream gradient gk.
-template <class M>
-struct Adj {
-  static constexpr bool ok = false;
-};
-
-// Chi2PDC: k = (c P) conj(y)  ->  g_y = (c P) conj(g_k),  g_m = y g_k
-template <>
-struct Adj<MChi2PDC> {
-  static constexpr bool ok = true;
-  template <int ST>
-  NLM_DEV static void fadj(const cplx& c, const cplx& y, const cplx& P, const cplx& gk, cplx& gy, cplx& gm) {
-    const cplx m = c * P;
-    gy = m * conj(gk);
-    gm = y * gk;
-  }
+  const cplx* coef;    // [nZ][3] step coefficients c = pol * nlStep * mismatch
+  const cplx* pump0;   // [2nZ-1][N] pump time grid
+  const cplx* tab;     // [nZ][N][2] RK4 step propagators (alpha, beta) / N
 };
 
 template <int L>
@@ -72,6 +62,9 @@ struct AdjLaunch {
   static constexpr int FA = T >= 256 ? 1 : 256 / T;  // fields per workgroup
   static constexpr int THREADS = FA * T;
   static constexpr int WAVES = (THREADS + 63) / 64;
+  // LDS: one FFT buffer per field plus, with several fields, a second row
+  // buffer per field for the workgroup sums of the pump-row gradients
+  static constexpr int LDS_ELEMS = (FA == 1 ? 1 : 2) * FA * FftShape<L>::lds_elems;
 };
 
 // wave-wide sum (xor butterfly; every lane ends with the total)
@@ -81,45 +74,63 @@ NLM_DEV double wave_sum(double v) {
   return v;
 }
 
-// one full forward split step i on the time-domain field (same code as k_prop)
-template <class M, int L>
-NLM_DEV void fwd_step(cplx (&a)[M::NS][8], int i, const AParams& ap, cplx* buf, int t, const FftTwiddles<L>& tw,
-                      const cplx (&ds)[M::NS][8]) {
-  constexpr int N = FftShape<L>::N, T = FftShape<L>::T, NS = M::NS, NC = M::NC;
-  const cplx* c = ap.coef + (size_t)i * 3 * NC;
+// One forward split step i from the scaled state a (time domain) with the
+// step propagator table: a <- IFFT(FFT(alpha a + beta conj(a)) exp(i dz D)).
+template <int L>
+NLM_DEV void fwd_step_tab(cplx (&a)[8], int i, const AParams& ap, cplx* buf, int t, const FftTwiddles<L>& tw) {
+  constexpr int N = FftShape<L>::N, T = FftShape<L>::T;
+  const cplx* row = ap.tab + (size_t)i * N * 2;
 #pragma unroll
   for (int m = 0; m < 8; m++) {
-    cplx x[NS], y[NS], k1[NS], k2[NS], k3[NS], k4[NS], P0[1], P1[1], P2[1];
-    P0[0] = ap.pump0[(size_t)(2 * i - 2) * N + t + T * m];
-    P1[0] = ap.pump0[(size_t)(2 * i - 1) * N + t + T * m];
-    P2[0] = ap.pump0[(size_t)(2 * i) * N + t + T * m];
-#pragma unroll
-    for (int q = 0; q < NS; q++) x[q] = a[q][m];
-    M::template f<0>(c, ap.cconst, x, P0, k1);
-#pragma unroll
-    for (int q = 0; q < NS; q++) y[q] = x[q] + 0.5 * k1[q];
-    M::template f<1>(c + NC, ap.cconst, y, P1, k2);
-#pragma unroll
-    for (int q = 0; q < NS; q++) y[q] = x[q] + 0.5 * k2[q];
-    M::template f<2>(c + NC, ap.cconst, y, P1, k3);
-#pragma unroll
-    for (int q = 0; q < NS; q++) y[q] = x[q] + k3[q];
-    M::template f<3>(c + 2 * NC, ap.cconst, y, P2, k4);
-#pragma unroll
-    for (int q = 0; q < NS; q++) a[q][m] = x[q] + (((k1[q] + 2.0 * k2[q]) + 2.0 * k3[q]) + k4[q]) * (1. / 6.);
+    const cplx al = row[(size_t)(t + T * m) * 2], be = row[(size_t)(t + T * m) * 2 + 1];
+    const cplx x = a[m] * (double)N;  // the table holds alpha / N, beta / N (exact scaling)
+    a[m] = mk(al.re * x.re - al.im * x.im + be.re * x.re + be.im * x.im,
+              al.re * x.im + al.im * x.re + be.im * x.re - be.re * x.im);
   }
+  fft_field<L, false>(a, buf, t, tw);
 #pragma unroll
-  for (int q = 0; q < NS; q++) {
-    fft_field<L, false>(a[q], buf, t, tw);
+  for (int m = 0; m < 8; m++) a[m] = a[m] * ap.dstep[t + T * m];
+  fft_field<L, true>(a, buf, t, tw);
+  scale_inv<L>(a);
+}
+
+// Sum two rows (E, O) over the workgroup's fields: every field writes both
+// into its LDS row slots; group 0 then reads the sums of E, group 1 (group 0
+// when there is one field per workgroup) those of O.
+template <int L>
+NLM_DEV void wg_sum_rows(const cplx (&e)[8], const cplx (&o)[8], cplx* lds, int lf, int t, cplx (&se)[8],
+                         cplx (&so)[8]) {
+  using F = FftShape<L>;
+  constexpr int T = F::T, E = F::lds_elems, FA = AdjLaunch<L>::FA;
+  if constexpr (FA == 1) {
 #pragma unroll
-    for (int m = 0; m < 8; m++) a[q][m] = a[q][m] * ds[q][m];
-    fft_field<L, true>(a[q], buf, t, tw);
-    scale_inv<L>(a[q]);
+    for (int m = 0; m < 8; m++) {
+      se[m] = e[m];
+      so[m] = o[m];
+    }
+  } else {
+    lds_barrier();
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      lds[lf * E + lpad(t + T * m)] = e[m];
+      lds[(FA + lf) * E + lpad(t + T * m)] = o[m];
+    }
+    lds_barrier();
+    if (lf < 2) {
+      const int base = lf == 0 ? 0 : FA * E;
+#pragma unroll
+      for (int m = 0; m < 8; m++) {
+        cplx s = lds[base + lpad(t + T * m)];
+        for (int f = 1; f < FA; f++) s = s + lds[base + f * E + lpad(t + T * m)];
+        if (lf == 0) se[m] = s;
+        else so[m] = s;
+      }
+    }
   }
 }
 
-// Sum a pump-row gradient over the workgroup's fields (through the fields' LDS
-// buffers) and write it to this workgroup's partial row.
+// Sum a pump-row gradient over the workgroup's fields (through LDS) and write
+// it to this workgroup's partial row.
 template <int L>
 NLM_DEV void emit_pump_row(const cplx (&v)[8], cplx* lds, int lf, int t, cplx* dst) {
   using F = FftShape<L>;
@@ -131,19 +142,18 @@ NLM_DEV void emit_pump_row(const cplx (&v)[8], cplx* lds, int lf, int t, cplx* d
   const int tau = lf * T + t;
   for (int k = tau; k < N; k += TH) {
     cplx s = lds[lpad(k)];
-#pragma unroll
     for (int f = 1; f < FA; f++) s = s + lds[f * F::lds_elems + lpad(k)];
     dst[k] = s;
   }
 }
 
-template <class M, int L>
-__global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint(AParams ap) {
+// Reverse sweep of the Chi2PDC Green's function (k = c P conj(y), one signal
+// mode, one pump): one field per T lanes, FA fields per workgroup.
+template <int L>
+__global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams ap) {
   using F = FftShape<L>;
-  constexpr int N = F::N, T = F::T, NS = M::NS, NC = M::NC;
-  constexpr int FA = AdjLaunch<L>::FA;
-  static_assert(NS == 1 && M::NP == 1, "adjoint implemented for single-mode single-pump models");
-  __shared__ cplx lds[FA * F::lds_elems];
+  constexpr int N = F::N, T = F::T, FA = AdjLaunch<L>::FA;
+  __shared__ cplx lds[AdjLaunch<L>::LDS_ELEMS];
   const int lf = threadIdx.x / T, t = threadIdx.x % T;
   const int field = blockIdx.x * FA + lf;
   const bool active = field < ap.nFields;
@@ -155,9 +165,6 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint(AParams ap) {
 
   FftTwiddles<L> tw;
   fft_load_twiddles<L>(tw, ap.tw, t);
-  cplx ds[NS][8];
-#pragma unroll
-  for (int m = 0; m < 8; m++) ds[0][m] = ap.dstep[t + T * m];
 
   // gradient of the last frequency row: output = F_last * halfOut (freq) or its IFFT (time)
   cplx g[8];
@@ -178,19 +185,41 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint(AParams ap) {
   for (int m = 0; m < 8; m++) carry[m] = mk(0, 0);
   cplx* gpumpWG = ap.gpump ? ap.gpump + (size_t)blockIdx.x * nZP * N : nullptr;
 
+  // envelope gradient accumulators: group 0 takes the even pump rows 2i,
+  // group 1 (group 0 if FA == 1) the odd rows 2i-1; ph = exp(-i r dzp D_p)
+  // for the group's next row r, advanced by exp(2 i dzp D_p) per step
+  const bool envAcc = ap.genvF != nullptr;
+  cplx acc[8], ph[8];
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    acc[m] = mk(0, 0);
+    ph[m] = mk(1, 0);
+  }
+  if (envAcc && lf < 2) {
+    const int r0 = lf == 0 ? 2 * (ap.nZ - 1) : 2 * (ap.nZ - 1) - 1;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      double sn, cs;
+      sincos((double)r0 * ap.dzp * ap.Dp[t + T * m], &sn, &cs);
+      ph[m] = mk(cs, -sn);
+    }
+  }
+
   cplx* segF = ap.seg + (size_t)fld * ap.K * N;
   for (int s = ap.nSeg - 1; s >= 0; s--) {
     const int i0 = s * ap.K;
     const int i1 = min(i0 + ap.K, ap.nZ - 1);
-    cplx a[NS][8];
+    {
+      cplx a[8];
 #pragma unroll
-    for (int m = 0; m < 8; m++) a[0][m] = ap.ckpt[((size_t)fld * ap.nSeg + s) * N + t + T * m];
-    for (int i = i0 + 1; i <= i1; i++) {
-      if (active) {
+      for (int m = 0; m < 8; m++) a[m] = ap.ckpt[((size_t)fld * ap.nSeg + s) * N + t + T * m];
+      for (int i = i0 + 1; i <= i1; i++) {
+        if (active) {
 #pragma unroll
-        for (int m = 0; m < 8; m++) segF[(size_t)(i - i0 - 1) * N + t + T * m] = a[0][m];
+          for (int m = 0; m < 8; m++) segF[(size_t)(i - i0 - 1) * N + t + T * m] = a[m];
+        }
+        if (i < i1) fwd_step_tab<L>(a, i, ap, buf, t, tw);
       }
-      if (i < i1) fwd_step<M, L>(a, i, ap, buf, t, tw, ds);
     }
     for (int i = i1; i > i0; i--) {
       // adjoint of the dispersion step: F = FFT(r) exp(i dz D); a_i = IFFT(F) (i < nZ-1)
@@ -199,15 +228,17 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint(AParams ap) {
         scale_inv<L>(g);
       }
 #pragma unroll
-      for (int m = 0; m < 8; m++) g[m] = conj(ds[0][m]) * g[m];
+      for (int m = 0; m < 8; m++) g[m] = conj(ap.dstep[t + T * m]) * g[m];
       fft_field<L, true>(g, buf, t, tw);  // g_r = N * IFFT(.) = unscaled inverse
       gFreq = false;
 
-      // adjoint of the RK4 step with the stored state a_{i-1}
-      const cplx* c = ap.coef + (size_t)i * 3 * NC;
-      const cplx c0 = c[0], c1 = c[NC], c2 = c[2 * NC];
+      // adjoint of the RK4 step (src/_NonlinearMedium.hpp:207-210 with
+      // Chi2PDC::DiffEq, solvers/Chi2PDC.cpp:27-47) at the stored state a_{i-1}:
+      // m_s = c_s P_s, k = m conj(y)  ->  g_y = m conj(g_k), g_m = y g_k
+      const cplx* c = ap.coef + (size_t)i * 3;
+      const cplx c0 = c[0], c1 = c[1], c2 = c[2];
       cplx gc0 = mk(0, 0), gc1 = mk(0, 0), gc2 = mk(0, 0);
-      cplx gP0[8], gP1[8], gP2[8];
+      cplx rowE[8], rowO[8];
 #pragma unroll
       for (int m = 0; m < 8; m++) {
         const int k = t + T * m;
@@ -215,64 +246,103 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint(AParams ap) {
         const cplx P0 = ap.pump0[(size_t)(2 * i - 2) * N + k];
         const cplx P1 = ap.pump0[(size_t)(2 * i - 1) * N + k];
         const cplx P2 = ap.pump0[(size_t)(2 * i) * N + k];
-        cplx xs[1] = {x}, ys[1], k1[1], k2[1], k3[1], Pa[1] = {P0}, Pb[1] = {P1};
-        M::template f<0>(c, ap.cconst, xs, Pa, k1);
-        const cplx y2 = x + 0.5 * k1[0];
-        ys[0] = y2;
-        M::template f<1>(c + NC, ap.cconst, ys, Pb, k2);
-        const cplx y3 = x + 0.5 * k2[0];
-        ys[0] = y3;
-        M::template f<2>(c + NC, ap.cconst, ys, Pb, k3);
-        const cplx y4 = x + k3[0];
+        const cplx m0 = c0 * P0, m1 = c1 * P1, m2 = c2 * P2;
+        const cplx y2 = x + 0.5 * (m0 * conj(x));
+        const cplx y3 = x + 0.5 * (m1 * conj(y2));
+        const cplx y4 = x + m1 * conj(y3);
         const cplx gr = g[m];
-        cplx gx = gr, gk1 = gr * (1. / 6.), gk2 = gr * (1. / 3.), gk3 = gr * (1. / 3.), gk4 = gr * (1. / 6.);
-        cplx gy, gm;
-        Adj<M>::template fadj<3>(c2, y4, P2, gk4, gy, gm);
+        const cplx gk4 = gr * (1. / 6.);
+        cplx gk3 = gr * (1. / 3.), gk2 = gr * (1. / 3.), gk1 = gk4, gx = gr, gy;
+        gy = m2 * conj(gk4);
+        const cplx gm2 = y4 * gk4;
         gx = gx + gy;
         gk3 = gk3 + gy;
-        gc2 = gc2 + conj(P2) * gm;
-        gP2[m] = conj(c2) * gm;
-        Adj<M>::template fadj<2>(c1, y3, P1, gk3, gy, gm);
+        gy = m1 * conj(gk3);
+        cplx gm1 = y3 * gk3;
         gx = gx + gy;
         gk2 = gk2 + 0.5 * gy;
-        gc1 = gc1 + conj(P1) * gm;
-        gP1[m] = conj(c1) * gm;
-        Adj<M>::template fadj<1>(c1, y2, P1, gk2, gy, gm);
+        gy = m1 * conj(gk2);
+        gm1 = gm1 + y2 * gk2;
         gx = gx + gy;
         gk1 = gk1 + 0.5 * gy;
-        gc1 = gc1 + conj(P1) * gm;
-        gP1[m] = gP1[m] + conj(c1) * gm;
-        Adj<M>::template fadj<0>(c0, x, P0, gk1, gy, gm);
+        gy = m0 * conj(gk1);
+        const cplx gm0 = x * gk1;
         gx = gx + gy;
-        gc0 = gc0 + conj(P0) * gm;
-        gP0[m] = conj(c0) * gm;
         g[m] = active ? gx : mk(0, 0);
-        if (!active) {
-          gP0[m] = gP1[m] = gP2[m] = mk(0, 0);
-        }
+        gc0 = gc0 + conj(P0) * gm0;
+        gc1 = gc1 + conj(P1) * gm1;
+        gc2 = gc2 + conj(P2) * gm2;
+        rowE[m] = conj(c2) * gm2 + carry[m];
+        rowO[m] = conj(c1) * gm1;
+        carry[m] = conj(c0) * gm0;
+        if (!active) rowE[m] = rowO[m] = carry[m] = mk(0, 0);
       }
       if (!active) gc0 = gc1 = gc2 = mk(0, 0);
       // per-wave partial of the coefficient gradients of step i
       const double r0 = wave_sum(gc0.re), i0r = wave_sum(gc0.im), r1 = wave_sum(gc1.re), i1r = wave_sum(gc1.im),
                    r2 = wave_sum(gc2.re), i2r = wave_sum(gc2.im);
       if (lane == 0) {
-        cplx* dst = ap.gcoef + ((size_t)wave * ap.nZ + i) * 3 * NC;
+        cplx* dst = ap.gcoef + ((size_t)wave * ap.nZ + i) * 3;
         dst[0] = mk(r0, i0r);
-        dst[NC] = mk(r1, i1r);
-        dst[2 * NC] = mk(r2, i2r);
+        dst[1] = mk(r1, i1r);
+        dst[2] = mk(r2, i2r);
       }
-      // pump rows: 2i (k4 of step i + k1 of step i+1), 2i-1 (k2, k3 of step i)
+      // pump rows 2i (k4 of step i + k1 of step i+1) and 2i-1 (k2, k3 of step i)
       if (gpumpWG) {
+        emit_pump_row<L>(rowE, lds, lf, t, gpumpWG + (size_t)(2 * i) * N);
+        emit_pump_row<L>(rowO, lds, lf, t, gpumpWG + (size_t)(2 * i - 1) * N);
+      }
+      if (envAcc) {
+        // g_F0 += s_r / N conj(exp(i r dzp D_p)) FFT(g_row r)   (rows r >= 1)
+        cplx se[8], so[8];
 #pragma unroll
-        for (int m = 0; m < 8; m++) gP2[m] = gP2[m] + carry[m];
-        emit_pump_row<L>(gP2, lds, lf, t, gpumpWG + (size_t)(2 * i) * N);
-        emit_pump_row<L>(gP1, lds, lf, t, gpumpWG + (size_t)(2 * i - 1) * N);
+        for (int m = 0; m < 8; m++) se[m] = so[m] = mk(0, 0);
+        wg_sum_rows<L>(rowE, rowO, lds, lf, t, se, so);
+        const double sE = (ap.strength ? ap.strength[2 * i] : 1.0) / N;
+        const double sO = (ap.strength ? ap.strength[2 * i - 1] : 1.0) / N;
+        if constexpr (FA == 1) {
+          fft_field<L, false>(se, buf, t, tw);
+          fft_field<L, false>(so, buf, t, tw);
 #pragma unroll
-        for (int m = 0; m < 8; m++) carry[m] = gP0[m];
+          for (int m = 0; m < 8; m++) {
+            const cplx p1 = ap.ph1[t + T * m], p2 = ap.ph2[t + T * m];
+            const cplx phO = ph[m] * p1;
+            acc[m] = acc[m] + sE * (ph[m] * se[m]) + sO * (phO * so[m]);
+            ph[m] = ph[m] * p2;
+          }
+        } else {
+          cplx v[8];
+#pragma unroll
+          for (int m = 0; m < 8; m++) v[m] = lf == 1 ? so[m] : se[m];
+          fft_field<L, false>(v, buf, t, tw);
+          const double sc = lf == 1 ? sO : sE;
+#pragma unroll
+          for (int m = 0; m < 8; m++) {
+            acc[m] = acc[m] + sc * (ph[m] * v[m]);
+            ph[m] = ph[m] * ap.ph2[t + T * m];
+          }
+        }
       }
     }
   }
   if (gpumpWG) emit_pump_row<L>(carry, lds, lf, t, gpumpWG);  // row 0
+  if (envAcc) {
+    cplx s0[8], unused[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) s0[m] = unused[m] = mk(0, 0);
+    wg_sum_rows<L>(carry, carry, lds, lf, t, s0, unused);
+    if (lf == 0) {
+#pragma unroll
+      for (int m = 0; m < 8; m++) ap.genvT[(size_t)blockIdx.x * N + t + T * m] = s0[m];
+    }
+    if (lf < 2) {
+#pragma unroll
+      for (int m = 0; m < 8; m++) {
+        ap.genvF[((size_t)blockIdx.x * 2 + lf) * N + t + T * m] = acc[m];
+        if (FA == 1) ap.genvF[((size_t)blockIdx.x * 2 + 1) * N + t + T * m] = mk(0, 0);
+      }
+    }
+  }
 
   // adjoint of the prologue: a_0 = IFFT(F0), F0 = in * halfIn (in = FFT(x) for time input)
   if (!gFreq) {
@@ -286,6 +356,18 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint(AParams ap) {
 #pragma unroll
     for (int m = 0; m < 8; m++) ap.gin[(size_t)fld * N + t + T * m] = g[m];
   }
+}
+
+// exp(i dzp D_p), exp(2 i dzp D_p)
+__global__ void k_pump_phase_steps(const double* __restrict__ Dp, double dzp, int N, cplx* __restrict__ ph1,
+                                   cplx* __restrict__ ph2) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  double sn, cs;
+  sincos(dzp * Dp[k], &sn, &cs);
+  ph1[k] = mk(cs, sn);
+  sincos(2.0 * dzp * Dp[k], &sn, &cs);
+  ph2[k] = mk(cs, sn);
 }
 
 // Adjoint of the Green's assembly: gradients of the final rows G(e_j), G(i e_j)
@@ -326,49 +408,4 @@ __global__ void k_sum_partials(const cplx* __restrict__ part, long n, int parts,
   cplx s = mk(0, 0);
   for (int p = 0; p < parts; p++) s = s + part[(size_t)p * n + k];
   out[k] = s;
-}
-
-// Adjoint of the linear pump grid for one pump: from the time-row gradients
-// gT[row] (already summed over the Green's fields), accumulate
-//   g_F0 += s_row conj(exp(i row dzp D)) FFT(gT[row]) / N   (rows >= 1)
-// per lane group over a strided set of rows; writes [groups][N] partials.
-template <int L>
-__global__ void __launch_bounds__(Launch<L>::THREADS)
-    k_pump_linear_adjoint(const cplx* __restrict__ gT, const double* __restrict__ Dp, double dzp,
-                          const double* __restrict__ strength, int nZP, cplx* __restrict__ partial,
-                          const cplx* __restrict__ twt) {
-  using F = FftShape<L>;
-  constexpr int N = F::N, T = F::T, FPB = Launch<L>::FPB;
-  __shared__ cplx lds[FPB * F::lds_elems];
-  const int lf = threadIdx.x / T, t = threadIdx.x % T;
-  const int group = blockIdx.x * FPB + lf;
-  const int groups = gridDim.x * FPB;
-  cplx* buf = lds + lf * F::lds_elems;
-  FftTwiddles<L> tw;
-  fft_load_twiddles<L>(tw, twt, t);
-  cplx acc[8];
-#pragma unroll
-  for (int m = 0; m < 8; m++) acc[m] = mk(0, 0);
-  // every lane group runs the same number of iterations (uniform barriers)
-  const int iters = (nZP - 1 + groups - 1) / groups;
-  for (int it = 0; it < iters; it++) {
-    const int row = 1 + group + it * groups;
-    const bool ok = row < nZP;
-    cplx v[8];
-#pragma unroll
-    for (int m = 0; m < 8; m++) v[m] = ok ? gT[(size_t)row * N + t + T * m] : mk(0, 0);
-    fft_field<L, false>(v, buf, t, tw);
-    if (ok) {
-      const double s = strength ? strength[row] : 1.0;
-      const double zi = (double)row * dzp;
-#pragma unroll
-      for (int m = 0; m < 8; m++) {
-        double sn, cs;
-        sincos(zi * Dp[t + T * m], &sn, &cs);
-        acc[m] = acc[m] + (s / N) * (mk(cs, -sn) * v[m]);
-      }
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < 8; m++) partial[(size_t)group * N + t + T * m] = acc[m];
 }

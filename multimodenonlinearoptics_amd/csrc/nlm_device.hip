@@ -595,14 +595,30 @@ static void input_channels(int nS, int nCh, int inputMode, int* inPos) {
   }
 }
 
-int launch_prop_kernel(nlm_plan* p, const KParams& kp) {
-  prop_fn fn = nlm_pick_prop(p->model, p->L);
+int launch_prop_kernel(nlm_plan* p, const KParams& kp0) {
+  KParams kp = kp0;
+  prop_fn fn = nullptr;
+  if (p->stepTable && (p->model == NLM_CHI2PDC || p->model == NLM_CHI3)) {
+    if (int e = p->stepTab.ensure((size_t)p->nZ * p->N * 2)) return e;
+    nlm_launch_step_table(p->model, p->coef.p, p->cconst.p, p->pumpT[0].p, p->N, p->nZ, p->stepTab.p, p->stream);
+    HIP_TRY(hipGetLastError());
+    kp.tab = p->stepTab.p;
+    fn = nlm_pick_prop_tab(p->L);
+  } else {
+    fn = nlm_pick_prop(p->model, p->L);
+  }
   if (!fn) return set_err(NLM_INVALID_ARGUMENT, "unsupported model / size");
   int fpb = 1, T = p->N / 8;
   if (T < 64) fpb = 64 / T;
   const int grid = (kp.nFields + fpb - 1) / fpb;
   fn(kp, grid, p->stream);
   HIP_TRY(hipGetLastError());
+  return NLM_OK;
+}
+
+extern "C" int nlm_plan_set_step_table(nlm_plan* p, int enable) {
+  if (!p) return set_err(NLM_INVALID_ARGUMENT, "null plan");
+  p->stepTable = enable ? 1 : 0;
   return NLM_OK;
 }
 

@@ -571,6 +571,7 @@ std::pair<std::vector<cd>, std::vector<double>> Medium::greensAdjoint(const Grid
 }
 
 void Medium::enableTiming(bool on) { check(nlm_plan_enable_timing(plan(), on ? 1 : 0)); }
+void Medium::setStepTable(bool on) { check(nlm_plan_set_step_table(plan(), on ? 1 : 0)); }
 
 std::vector<double> Medium::lastTiming() {
   std::vector<double> ms(3, 0.0);

@@ -101,6 +101,7 @@ class Medium {
   std::pair<Grid, Grid> greensForward(bool inTimeDomain, bool runPump, int ckptEvery);
   std::pair<std::vector<cd>, std::vector<double>> greensAdjoint(const Grid& gC, const Grid& gS);
   void enableTiming(bool on);
+  void setStepTable(bool on);
   std::vector<double> lastTiming();
   void sync();
 

@@ -61,6 +61,8 @@ struct nlm_plan_s {
   double dzp = 0;
   DevBuf<cplx> ckpt, seg, gG, gC, gS, gcoefPart, gcoef, gpumpPart, gpump, gF0Part, gF0, gtmp;
   DevBuf<cplx> scale;
+  DevBuf<cplx> stepTab;  // RK4 step propagators of one-mode undepleted media [nZ][N][2]
+  int stepTable = 1;      // use them (nlm_plan_set_step_table)
   int ckptEvery = 0, nSeg = 0, gNIn = 0, gNOut = 0, gInTime = 0, gHasScale = 0;
 };
 

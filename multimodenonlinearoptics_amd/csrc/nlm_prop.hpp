@@ -49,6 +49,7 @@ struct KParams {
   const cplx* filt;
   cplx* ckpt;          // optional: time-domain field every ckptEvery steps, [nFields][nSeg][NS][N]
   int ckptEvery, nSeg;
+  const cplx* tab;     // k_prop_tab: RK4 step propagators [nZ][N][2] (alpha, beta) / N
 };
 
 template <int L>
@@ -387,3 +388,7 @@ static void launch_gnlse(const KParams& kp, int grid, hipStream_t s) {
 
 // Launcher of model MODEL for log2 N = L (3..13), or nullptr.
 prop_fn nlm_pick_prop(int model, int L);
+// Step-propagator kernel (one signal mode) for log2 N = L, and its table builder.
+prop_fn nlm_pick_prop_tab(int L);
+bool nlm_launch_step_table(int model, const cplx* coef, const cplx* cconst, const cplx* pump, int N, int nZ,
+                           cplx* tab, hipStream_t s);

@@ -1,0 +1,165 @@
+// nlm_prop_tab.hip -- split step with a precomputed RK4 step propagator for the
+// single-signal-mode undepleted media (Chi3, Chi2PDC).
+//
+// With the pump grid fixed, the DiffEq of these media is R-linear in the
+// signal: Chi2PDC (solvers/Chi2PDC.cpp:27-47) has k = (c P) conj(x), Chi3
+// (solvers/Chi3.cpp:57-69) has k = i nl (2|P|^2 x + P^2 conj(x)).  Every RK4
+// stage (src/_NonlinearMedium.hpp:207-210) is then R-linear too, and one
+// whole RK4 step at time sample k of step i is
+//     x -> alpha[i][k] x + beta[i][k] conj(x),
+// with alpha = (R(1) - i R(i)) / 2 and beta = (R(1) + i R(i)) / 2 for R the
+// RK4 step applied to the inputs 1 and i.  k_step_table evaluates R with the
+// model's own DiffEq functor once per (step, sample) for all fields; the
+// propagation kernel then spends 8 FP64 FMAs per point on the nonlinear step
+// instead of four DiffEq evaluations, for every one of the 2N Green's impulses
+// or batch rows.  The propagator also absorbs the inverse transform's 1/N
+// (an exact power of two), so the field between steps is kept unscaled.
+#include "nlm_prop.hpp"
+
+template <class M>
+__device__ __forceinline__ cplx rk4_step(const cplx* c, const cplx* cc, cplx x, cplx P0, cplx P1, cplx P2) {
+  constexpr int NC = M::NC;
+  cplx xs[1] = {x}, y[1], k1[1], k2[1], k3[1], k4[1], p0[1] = {P0}, p1[1] = {P1}, p2[1] = {P2};
+  M::template f<0>(c, cc, xs, p0, k1);
+  y[0] = x + 0.5 * k1[0];
+  M::template f<1>(c + NC, cc, y, p1, k2);
+  y[0] = x + 0.5 * k2[0];
+  M::template f<2>(c + NC, cc, y, p1, k3);
+  y[0] = x + k3[0];
+  M::template f<3>(c + 2 * NC, cc, y, p2, k4);
+  return x + (((k1[0] + 2.0 * k2[0]) + 2.0 * k3[0]) + k4[0]) * (1. / 6.);
+}
+
+// tab[(i*N + k)*2 + {0,1}] = {alpha, beta} * scale for steps i = 1..nZ-1
+template <class M>
+__global__ void k_step_table(const cplx* __restrict__ coef, const cplx* __restrict__ cconst,
+                             const cplx* __restrict__ pump, int N, int nZ, double scale, cplx* __restrict__ tab) {
+  static_assert(M::NS == 1 && M::NP == 1, "single signal mode, single pump");
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)(nZ - 1) * N;
+  if (idx >= total) return;
+  const int i = 1 + (int)(idx / N), k = (int)(idx % N);
+  const cplx* c = coef + (size_t)i * 3 * M::NC;
+  const cplx P0 = pump[(size_t)(2 * i - 2) * N + k];
+  const cplx P1 = pump[(size_t)(2 * i - 1) * N + k];
+  const cplx P2 = pump[(size_t)(2 * i) * N + k];
+  const cplx r1 = rk4_step<M>(c, cconst, mk(1.0, 0.0), P0, P1, P2);
+  const cplx ri = rk4_step<M>(c, cconst, mk(0.0, 1.0), P0, P1, P2);
+  const cplx iri = mk(-ri.im, ri.re);  // i * R(i)
+  tab[((size_t)i * N + k) * 2] = (0.5 * scale) * (r1 - iri);
+  tab[((size_t)i * N + k) * 2 + 1] = (0.5 * scale) * (r1 + iri);
+}
+
+template <int L>
+__device__ __forceinline__ void load_prop_row(cplx (&al)[8], cplx (&be)[8], const cplx* __restrict__ row, int t) {
+  constexpr int T = FftShape<L>::T;
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    al[m] = row[(size_t)(t + T * m) * 2];
+    be[m] = row[(size_t)(t + T * m) * 2 + 1];
+  }
+}
+
+template <int L>
+__device__ __forceinline__ void store_scaled(cplx* __restrict__ dst, const cplx (&v)[8], int t, bool active) {
+  constexpr int T = FftShape<L>::T;
+  constexpr double s = 1.0 / FftShape<L>::N;
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < 8; m++) dst[t + T * m] = v[m] * s;
+  }
+}
+
+// Same contract as k_prop (Green's, batch and grid kinds, checkpoints) for a
+// medium with one signal mode and a step propagator table kp.tab.
+template <int L>
+__global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T, FPB = Launch<L>::FPB;
+  __shared__ cplx lds[FPB * F::lds_elems];
+
+  const int lf = threadIdx.x / T, t = threadIdx.x % T;
+  const int field = blockIdx.x * FPB + lf;
+  const bool active = field < kp.nFields;
+  cplx* buf = lds + lf * F::lds_elems;
+
+  FftTwiddles<L> tw;
+  fft_load_twiddles<L>(tw, kp.tw, t);
+  cplx ds[8];
+#pragma unroll
+  for (int m = 0; m < 8; m++) ds[m] = kp.dstep[t + T * m];
+
+  cplx a[1][8];
+  load_inputs<L, 1>(a, kp, field, t, active);
+  prologue<L, 1>(a, kp, buf, t, tw, active);
+  cplx al[8], be[8];
+  if (kp.nZ > 1) {
+    fft_field<L, true>(a[0], buf, t, tw);  // unscaled: the table holds the 1/N
+    if (kp.kind == KIND_GRID) store_scaled<L>(kp.gridT, a[0], t, active);
+    if (kp.ckpt) store_scaled<L>(kp.ckpt + (size_t)field * kp.nSeg * N, a[0], t, active);
+    load_prop_row<L>(al, be, kp.tab + (size_t)N * 2, t);
+  }
+
+  for (int i = 1; i < kp.nZ; i++) {
+    // --- RK4 nonlinear step as its R-linear propagator
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      const cplx x = a[0][m];
+      a[0][m] = mk(al[m].re * x.re - al[m].im * x.im + be[m].re * x.re + be[m].im * x.im,
+                   al[m].re * x.im + al[m].im * x.re + be[m].im * x.re - be[m].re * x.im);
+    }
+    if (i + 1 < kp.nZ) load_prop_row<L>(al, be, kp.tab + (size_t)(i + 1) * N * 2, t);
+
+    // --- dispersion step (src/_NonlinearMedium.hpp:212-215)
+    fft_field<L, false>(a[0], buf, t, tw);
+#pragma unroll
+    for (int m = 0; m < 8; m++) a[0][m] = a[0][m] * ds[m];
+    if (i < kp.nZ - 1) {
+      if (kp.kind == KIND_GRID) store_row<L>(kp.gridF + (size_t)i * N, a[0], t, active);
+      fft_field<L, true>(a[0], buf, t, tw);
+      if (kp.kind == KIND_GRID) store_scaled<L>(kp.gridT + (size_t)i * N, a[0], t, active);
+      if (kp.ckpt && i % kp.ckptEvery == 0)
+        store_scaled<L>(kp.ckpt + ((size_t)field * kp.nSeg + i / kp.ckptEvery) * N, a[0], t, active);
+    }
+  }
+  epilogue<L, 1>(a, kp, buf, t, tw, field, active);
+}
+
+template <int L>
+static void launch_prop_tab(const KParams& kp, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_prop_tab<L>), dim3(grid), dim3(Launch<L>::THREADS), 0, s, kp);
+}
+
+prop_fn nlm_pick_prop_tab(int L) {
+  switch (L) {
+    case 3: return &launch_prop_tab<3>;
+    case 4: return &launch_prop_tab<4>;
+    case 5: return &launch_prop_tab<5>;
+    case 6: return &launch_prop_tab<6>;
+    case 7: return &launch_prop_tab<7>;
+    case 8: return &launch_prop_tab<8>;
+    case 9: return &launch_prop_tab<9>;
+    case 10: return &launch_prop_tab<10>;
+    case 11: return &launch_prop_tab<11>;
+    case 12: return &launch_prop_tab<12>;
+    case 13: return &launch_prop_tab<13>;
+  }
+  return nullptr;
+}
+
+// Build the step propagator table of `model` (NLM_CHI3 or NLM_CHI2PDC) into tab
+// ([nZ][N][2] complex; row 0 unused).  Returns false for other models.
+bool nlm_launch_step_table(int model, const cplx* coef, const cplx* cconst, const cplx* pump, int N, int nZ,
+                           cplx* tab, hipStream_t s) {
+  const long total = (long)(nZ - 1) * N;
+  if (total <= 0) return true;
+  const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+  const double scale = 1.0 / N;
+  if (model == NLM_CHI2PDC)
+    hipLaunchKernelGGL(k_step_table<MChi2PDC>, grid, block, 0, s, coef, cconst, pump, N, nZ, scale, tab);
+  else if (model == NLM_CHI3)
+    hipLaunchKernelGGL(k_step_table<MChi3>, grid, block, 0, s, coef, cconst, pump, N, nZ, scale, tab);
+  else
+    return false;
+  return true;
+}

@@ -260,6 +260,9 @@ PYBIND11_MODULE(nonlinearmedium, m) {
            "Gradients (dL/d envelope, dL/d poling profile) of the last greensForward for dL/dC, dL/dS "
            "(convention dL/dRe + i dL/dIm) (extension; Chi2PDC).", "gC"_a, "gS"_a);
   base.def("enableTiming", &Medium::enableTiming, "Record HIP-event kernel times of each run (extension).", "on"_a);
+  base.def("setStepTable", &Medium::setStepTable,
+           "One-mode undepleted media: propagate with precomputed RK4 step propagators (default) or evaluate "
+           "the DiffEq per field (extension).", "on"_a);
   base.def("lastTiming", &Medium::lastTiming,
            "Kernel times of the last run in ms: [pump, propagation, assembly] (extension).");
   base.def("sync", [](Medium& md) {

@@ -152,3 +152,23 @@ def config2(mod, N=512, nZ=1000):
     tau = 2 * kw["tMax"] / N * np.fft.fftshift(np.arange(-N // 2, N // 2))
     kw["customPump"] = np.exp(-tau ** 2 / (2 * w * w)).astype(np.complex128)
     return mod.Chi2PDC(**kw)
+
+
+def config4(mod, N=1024, nZ=2000):
+    """Differentiable Chi2PDC: N=1024, nZ=2000, tMax=20, Gaussian pump (w=1) with a
+    seeded spectral phase phi ~ N(0, 0.5^2), per-step poling p ~ U(-1, 1).
+
+    Returns (medium, spectrum0, phi, poling): the pump envelope is
+    IFFT(spectrum0 * exp(i phi)) with spectrum0 = FFT(exp(-tau^2/2))."""
+    r = np.random.default_rng(SEED0 + 4)
+    b1s = float(r.uniform(-2, 2))
+    b2, b2s = (float(v) for v in r.uniform(-1, 1, size=2))
+    phi = r.normal(0.0, 0.5, size=N)
+    pol = r.uniform(-1.0, 1.0, size=nZ)
+    tMax = 20.0
+    tau = 2 * tMax / N * np.fft.fftshift(np.arange(-N // 2, N // 2))
+    spec0 = np.fft.fft(np.exp(-tau ** 2 / 2).astype(np.complex128))
+    env = np.fft.ifft(spec0 * np.exp(1j * phi))
+    m = mod.Chi2PDC(relativeLength=1.0, nlLength=1.0, beta2=b2, beta2s=b2s, beta1=0.0, beta1s=b1s, diffBeta0=0.0,
+                    tMax=tMax, tPrecision=N, nZSteps=nZ, customPump=env, polingProfile=list(pol))
+    return m, spec0, phi, pol

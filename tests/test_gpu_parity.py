@@ -154,3 +154,26 @@ def test_config1_chi3_full_grids(nlm, oracle):
     for a, b in ((dev.pumpTime, ref.pumpTime), (dev.pumpFreq, ref.pumpFreq), (dev.signalTime, ref.signalTime),
                  (dev.signalFreq, ref.signalFreq)):
         assert rel_l2(a, b) < TOL
+
+
+@pytest.mark.parametrize("name", ["Chi3", "Chi2PDC"])
+@pytest.mark.parametrize("table", [True, False])
+@pytest.mark.parametrize("N,nZ", [(8, 2), (64, 3), (256, 25), (8192, 6)])
+def test_step_table_and_direct_paths(nlm, oracle, name, table, N, nZ):
+    """One-mode undepleted media run with the RK4 step-propagator table (default)
+    or with the DiffEq evaluated per field; both match the oracle in every run kind."""
+    _, kw = small_case(name, nlm, N=N, nZ=nZ, seed=N + nZ, poled=nZ > 20)
+    _, kwo = small_case(name, oracle, N=N, nZ=nZ, seed=N + nZ, poled=nZ > 20)
+    dev = make(nlm, name, kw)
+    ref = make(oracle, name, kwo)
+    dev.setStepTable(table)
+    if N <= 256:
+        C, S = dev.computeGreensFunction(inTimeDomain=True)
+        Co, So = ref.computeGreensFunction(inTimeDomain=True)
+        assert rel_l2(np.hstack([C, S]), np.hstack([Co, So])) < TOL
+    X = random_inputs(name, N, 3, seed=2)
+    assert rel_l2(dev.batchSignalSimulation(X), ref.batchSignalSimulation(X)) < TOL
+    dev.runSignalSimulation(X[0], True)
+    ref.runSignalSimulation(X[0], True)
+    assert rel_l2(dev.signalTimes(0), ref.signalTimes(0)) < TOL
+    assert rel_l2(dev.signalFreqs(0), ref.signalFreqs(0)) < TOL
