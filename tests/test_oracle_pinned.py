@@ -268,3 +268,92 @@ def test_fully_nonlinear_has_no_pump_methods(oracle):
     for call in (m.runPumpSimulation, m.computeGreensFunction, lambda: m.setPump(0)):
         with pytest.raises(RuntimeError, match="Object does not have this method."):
             call()
+
+
+# ---------------------------------------------------------------------------
+# Manley-Rowe invariants of the fully nonlinear chi(2) media.  With coupling
+# constants kappa_p = 1/L_NL[p] the flows conserve weighted photon numbers
+# pointwise in time for any unit-modulus phase mismatch, poling sign and
+# beam-profile factor, and the dispersion step conserves each mode's energy,
+# so the sums below drift only by RK4 truncation.  Each check fails if a
+# coupling constant lands on the wrong mode or the factor 1/2 of the
+# second-harmonic term moves.
+# ---------------------------------------------------------------------------
+
+def _energies(m, modes):
+    return [float(np.sum(np.abs(np.asarray(m.signalTimes(q))) ** 2, axis=1)[[0, -1]].tolist()[0]) for q in modes], \
+        [float(np.sum(np.abs(np.asarray(m.signalTimes(q))[-1]) ** 2)) for q in modes]
+
+
+def _run_fnlm(m, nmodes, seed, amp=0.6):
+    r = np.random.default_rng(seed)
+    N = np.asarray(m.tau).size
+    tau = np.asarray(m.tau)
+    x = np.concatenate([amp * np.exp(-tau ** 2 / 2) * np.exp(1j * r.uniform(0, 6, N)) for _ in range(nmodes)])
+    m.runSignalSimulation(x, inTimeDomain=True)
+    first = [float(np.sum(np.abs(np.asarray(m.signalTimes(q))[0]) ** 2)) for q in range(nmodes)]
+    last = [float(np.sum(np.abs(np.asarray(m.signalTimes(q))[-1]) ** 2)) for q in range(nmodes)]
+    return np.array(first), np.array(last)
+
+
+@pytest.mark.parametrize("name", ["Chi2SHG", "Chi2SHGXPM", "Chi2ASHG"])
+def test_shg_manley_rowe(oracle, name):
+    LH, LP = 0.7, 1.9
+    kw = dict(relativeLength=1.0, nlLengthH=LH, nlLengthP=LP, beta2h=0.3, beta2p=-0.2, beta1h=0.4, beta1p=-0.1,
+              tMax=10, tPrecision=64, nZSteps=800, rayleighLength=1.3)
+    if name == "Chi2SHGXPM":
+        kw["nlLengthChi3"] = 2.0
+    if name == "Chi2ASHG":
+        kw.update(diffBeta0Start=-2.0, diffBeta0End=3.0)
+    else:
+        kw["diffBeta0"] = 1.5
+    m = getattr(oracle, name)(**kw)
+    e0, e1 = _run_fnlm(m, 2, seed=4)
+    # k[0] = i dz/L_P (...) conj(A) B,  k[1] = i dz/L_H (...) A^2 / 2   ->  |A|^2/L_H + 2|B|^2/L_P
+    q0 = e0[0] / LH + 2 * e0[1] / LP
+    q1 = e1[0] / LH + 2 * e1[1] / LP
+    assert abs(q1 - q0) <= 1e-9 * q0
+    assert abs(e1[1] - e0[1]) > 1e-3 * q0  # real conversion happened
+    swapped = (e1[0] / LP + 2 * e1[1] / LH) - (e0[0] / LP + 2 * e0[1] / LH)
+    assert abs(swapped) > 1e-4 * q0
+
+
+def test_dsfg_manley_rowe(oracle):
+    Lp, Ls, Ld = 0.8, 1.3, 2.1
+    m = oracle.Chi2DSFG(relativeLength=1.0, nlLengthP=Lp, nlLengthS=Ls, nlLengthD=Ld, beta2p=0.2, beta2s=-0.3,
+                        beta2d=0.1, beta1p=0.2, beta1s=0.0, beta1d=-0.3, diffBeta0=0.9, tMax=10, tPrecision=64,
+                        nZSteps=800)
+    e0, e1 = _run_fnlm(m, 3, seed=6)
+    kp, ks, kd = 1 / Lp, 1 / Ls, 1 / Ld
+    for a, b, ka, kb in ((0, 1, ks, kp), (1, 2, kd, ks)):
+        q0 = ka * e0[a] + kb * e0[b]
+        q1 = ka * e1[a] + kb * e1[b]
+        assert abs(q1 - q0) <= 1e-9 * q0
+    assert np.abs(e1 - e0).max() > 1e-3 * e0.sum()
+
+
+# ---------------------------------------------------------------------------
+# Cascade (src/Cascade.cpp:110-135): the composed Green's function applied to
+# an input equals the signal propagated through the media one after another
+# (src/Cascade.cpp:91-107).  Composing in the wrong order or conjugating the
+# wrong factor fails this.
+# ---------------------------------------------------------------------------
+
+def test_cascade_composition_equals_sequential_propagation(oracle):
+    N, nZ = 64, 50
+    a = oracle.Chi2PDC(relativeLength=1.0, nlLength=0.8, beta2=0.3, beta2s=-0.4, beta1=0.2, beta1s=-0.5,
+                       diffBeta0=0.7, tMax=10, tPrecision=N, nZSteps=nZ)
+    b = oracle.Chi2PDC(relativeLength=0.7, nlLength=1.0, beta2=-0.2, beta2s=0.5, beta1=-0.3, beta1s=0.4,
+                       diffBeta0=-1.2, tMax=10, tPrecision=N, nZSteps=nZ + 7)
+    cas = oracle.Cascade([a, b], [{0: 0}], False)
+    C, S = cas.computeGreensFunction()
+    r = np.random.default_rng(9)
+    x = (r.standard_normal(N) + 1j * r.standard_normal(N)) * np.exp(-np.asarray(a.tau) ** 2 / 8)
+    cas.runSignalSimulation(x, inTimeDomain=True)
+    xf = np.fft.fft(x)
+    pred = np.fft.ifftshift(C @ np.fft.fftshift(xf) + S @ np.fft.fftshift(xf.conj()))
+    assert rel_l2(pred, b.signalFreq[-1]) < 1e-12
+    # the reversed composition is a different operator
+    Ca, Sa = a.computeGreensFunction(runPump=False)
+    Cb, Sb = b.computeGreensFunction(runPump=False)
+    assert rel_l2(Ca @ Cb + Sa @ Sb.conj(), C) > 1e-3
