@@ -99,6 +99,7 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6):
     g_phi = np.real(np.conj(gX) * 1j * X)
     r = np.random.default_rng(SEED0 + 5)
     worst = 0.0
+    pairs = []
     for _ in range(n_dirs):
         d = r.standard_normal(N + nZ)
         d /= np.linalg.norm(d)
@@ -107,6 +108,7 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6):
             2 * fd_eps)
         ad = float(np.dot(g_phi, dphi) + np.dot(g_pol, dpol))
         worst = max(worst, abs(ad - fd) / max(abs(fd), 1e-30))
+        pairs.append((ad, fd))
     m.setPump(env_of(phi))
     m.setPolingProfile(pol)
     return {
@@ -118,6 +120,7 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6):
         "grad_vs_forward": min(t_bwd) / min(t_fwd),
         "grad_vs_forward_kernel": min(k_bwd) / min(k_fwd),
         "fd_directions": n_dirs, "fd_eps": fd_eps, "fd_max_rel_err": worst,
+        "fd_pairs": pairs[:6], "grad_norm": float(np.sqrt(np.sum(g_phi ** 2) + np.sum(g_pol ** 2))),
     }
 
 

@@ -79,7 +79,7 @@ def build(force=False, jobs=None, verbose=False):
                                                 os.path.join(CSRC, "nlm_prop_inst.hip"), "-o", o])
     host_srcs = ["nlm_host.cpp", "nlm_pybind.cpp"]
     import pybind11
-    host_flags = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + CSRC,
+    host_flags = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-I" + INCLUDE, "-I" + CSRC,
                   "-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"],
                   "-fvisibility=hidden"]
     host_objs = []

@@ -62,9 +62,11 @@ struct AdjLaunch {
   static constexpr int FA = T >= 256 ? 1 : 256 / T;  // fields per workgroup
   static constexpr int THREADS = FA * T;
   static constexpr int WAVES = (THREADS + 63) / 64;
-  // LDS: one FFT buffer per field plus, with several fields, a second row
-  // buffer per field for the workgroup sums of the pump-row gradients
-  static constexpr int LDS_ELEMS = (FA == 1 ? 1 : 2) * FA * FftShape<L>::lds_elems;
+  // LDS: one FFT buffer per field; with several fields also a second row
+  // buffer per field (the pump-row gradients are summed over the fields through
+  // the two) and the envelope-gradient accumulators and phases of groups 0, 1
+  static constexpr int LDS_ELEMS = (FA == 1 ? 1 : 2 * FA + 4) * FftShape<L>::lds_elems;
+  static constexpr int ACC_OFF = 2 * FA * FftShape<L>::lds_elems;
 };
 
 // wave-wide sum (xor butterfly; every lane ends with the total)
@@ -92,41 +94,6 @@ NLM_DEV void fwd_step_tab(cplx (&a)[8], int i, const AParams& ap, cplx* buf, int
   for (int m = 0; m < 8; m++) a[m] = a[m] * ap.dstep[t + T * m];
   fft_field<L, true>(a, buf, t, tw);
   scale_inv<L>(a);
-}
-
-// Sum two rows (E, O) over the workgroup's fields: every field writes both
-// into its LDS row slots; group 0 then reads the sums of E, group 1 (group 0
-// when there is one field per workgroup) those of O.
-template <int L>
-NLM_DEV void wg_sum_rows(const cplx (&e)[8], const cplx (&o)[8], cplx* lds, int lf, int t, cplx (&se)[8],
-                         cplx (&so)[8]) {
-  using F = FftShape<L>;
-  constexpr int T = F::T, E = F::lds_elems, FA = AdjLaunch<L>::FA;
-  if constexpr (FA == 1) {
-#pragma unroll
-    for (int m = 0; m < 8; m++) {
-      se[m] = e[m];
-      so[m] = o[m];
-    }
-  } else {
-    lds_barrier();
-#pragma unroll
-    for (int m = 0; m < 8; m++) {
-      lds[lf * E + lpad(t + T * m)] = e[m];
-      lds[(FA + lf) * E + lpad(t + T * m)] = o[m];
-    }
-    lds_barrier();
-    if (lf < 2) {
-      const int base = lf == 0 ? 0 : FA * E;
-#pragma unroll
-      for (int m = 0; m < 8; m++) {
-        cplx s = lds[base + lpad(t + T * m)];
-        for (int f = 1; f < FA; f++) s = s + lds[base + f * E + lpad(t + T * m)];
-        if (lf == 0) se[m] = s;
-        else so[m] = s;
-      }
-    }
-  }
 }
 
 // Sum a pump-row gradient over the workgroup's fields (through LDS) and write
@@ -188,20 +155,25 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
   // envelope gradient accumulators: group 0 takes the even pump rows 2i,
   // group 1 (group 0 if FA == 1) the odd rows 2i-1; ph = exp(-i r dzp D_p)
   // for the group's next row r, advanced by exp(2 i dzp D_p) per step
+  // (registers when FA == 1, LDS otherwise)
+  constexpr int E = F::lds_elems;
   const bool envAcc = ap.genvF != nullptr;
-  cplx acc[8], ph[8];
-#pragma unroll
-  for (int m = 0; m < 8; m++) {
-    acc[m] = mk(0, 0);
-    ph[m] = mk(1, 0);
-  }
+  cplx acc[FA == 1 ? 8 : 1], ph[FA == 1 ? 8 : 1];
+  cplx* accL = lds + AdjLaunch<L>::ACC_OFF + (lf & 1) * 2 * E;
+  cplx* phL = accL + E;
   if (envAcc && lf < 2) {
     const int r0 = lf == 0 ? 2 * (ap.nZ - 1) : 2 * (ap.nZ - 1) - 1;
 #pragma unroll
     for (int m = 0; m < 8; m++) {
       double sn, cs;
-      sincos((double)r0 * ap.dzp * ap.Dp[t + T * m], &sn, &cs);
-      ph[m] = mk(cs, -sn);
+      sincos_reduced((double)r0 * ap.dzp * ap.Dp[t + T * m], &sn, &cs);
+      if constexpr (FA == 1) {
+        acc[m] = mk(0, 0);
+        ph[m] = mk(cs, -sn);
+      } else {
+        accL[lpad(t + T * m)] = mk(0, 0);
+        phL[lpad(t + T * m)] = mk(cs, -sn);
+      }
     }
   }
 
@@ -238,7 +210,13 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
       const cplx* c = ap.coef + (size_t)i * 3;
       const cplx c0 = c[0], c1 = c[1], c2 = c[2];
       cplx gc0 = mk(0, 0), gc1 = mk(0, 0), gc2 = mk(0, 0);
-      cplx rowE[8], rowO[8];
+      // pump-row gradients of this step: registers (FA == 1) or the field's two
+      // LDS row buffers, free once the transform above has finished
+      cplx rowE[FA == 1 ? 8 : 1], rowO[FA == 1 ? 8 : 1];
+      const bool rows = gpumpWG || envAcc;
+      if constexpr (FA > 1) {
+        if (rows) lds_barrier();
+      }
 #pragma unroll
       for (int m = 0; m < 8; m++) {
         const int k = t + T * m;
@@ -272,10 +250,19 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
         gc0 = gc0 + conj(P0) * gm0;
         gc1 = gc1 + conj(P1) * gm1;
         gc2 = gc2 + conj(P2) * gm2;
-        rowE[m] = conj(c2) * gm2 + carry[m];
-        rowO[m] = conj(c1) * gm1;
-        carry[m] = conj(c0) * gm0;
-        if (!active) rowE[m] = rowO[m] = carry[m] = mk(0, 0);
+        const cplx re = active ? conj(c2) * gm2 + carry[m] : mk(0, 0);
+        const cplx ro = active ? conj(c1) * gm1 : mk(0, 0);
+        carry[m] = active ? conj(c0) * gm0 : mk(0, 0);
+        if constexpr (FA == 1) {
+          rowE[m] = re;
+          rowO[m] = ro;
+        } else if (rows) {
+          lds[lf * E + lpad(k)] = re;
+          lds[(FA + lf) * E + lpad(k)] = ro;
+        }
+      }
+      if constexpr (FA > 1) {
+        if (rows) lds_barrier();
       }
       if (!active) gc0 = gc1 = gc2 = mk(0, 0);
       // per-wave partial of the coefficient gradients of step i
@@ -289,37 +276,62 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
       }
       // pump rows 2i (k4 of step i + k1 of step i+1) and 2i-1 (k2, k3 of step i)
       if (gpumpWG) {
-        emit_pump_row<L>(rowE, lds, lf, t, gpumpWG + (size_t)(2 * i) * N);
-        emit_pump_row<L>(rowO, lds, lf, t, gpumpWG + (size_t)(2 * i - 1) * N);
+        cplx* dE = gpumpWG + (size_t)(2 * i) * N;
+        cplx* dO = gpumpWG + (size_t)(2 * i - 1) * N;
+        if constexpr (FA == 1) {
+#pragma unroll
+          for (int m = 0; m < 8; m++) {
+            dE[t + T * m] = rowE[m];
+            dO[t + T * m] = rowO[m];
+          }
+        } else {
+          for (int k = threadIdx.x; k < N; k += AdjLaunch<L>::THREADS) {
+            cplx se = mk(0, 0), so = mk(0, 0);
+            for (int f = 0; f < FA; f++) {
+              se = se + lds[f * E + lpad(k)];
+              so = so + lds[(FA + f) * E + lpad(k)];
+            }
+            dE[k] = se;
+            dO[k] = so;
+          }
+        }
       }
       if (envAcc) {
         // g_F0 += s_r / N conj(exp(i r dzp D_p)) FFT(g_row r)   (rows r >= 1)
-        cplx se[8], so[8];
-#pragma unroll
-        for (int m = 0; m < 8; m++) se[m] = so[m] = mk(0, 0);
-        wg_sum_rows<L>(rowE, rowO, lds, lf, t, se, so);
         const double sE = (ap.strength ? ap.strength[2 * i] : 1.0) / N;
         const double sO = (ap.strength ? ap.strength[2 * i - 1] : 1.0) / N;
         if constexpr (FA == 1) {
-          fft_field<L, false>(se, buf, t, tw);
-          fft_field<L, false>(so, buf, t, tw);
+          fft_field<L, false>(rowE, buf, t, tw);
+          fft_field<L, false>(rowO, buf, t, tw);
 #pragma unroll
           for (int m = 0; m < 8; m++) {
             const cplx p1 = ap.ph1[t + T * m], p2 = ap.ph2[t + T * m];
             const cplx phO = ph[m] * p1;
-            acc[m] = acc[m] + sE * (ph[m] * se[m]) + sO * (phO * so[m]);
+            acc[m] = acc[m] + sE * (ph[m] * rowE[m]) + sO * (phO * rowO[m]);
             ph[m] = ph[m] * p2;
           }
         } else {
+          // group 0 transforms the workgroup sum of row 2i, group 1 that of
+          // row 2i-1; the other groups transform zeros (uniform barriers)
           cplx v[8];
-#pragma unroll
-          for (int m = 0; m < 8; m++) v[m] = lf == 1 ? so[m] : se[m];
-          fft_field<L, false>(v, buf, t, tw);
-          const double sc = lf == 1 ? sO : sE;
+          const int base = lf == 0 ? 0 : FA * E;
 #pragma unroll
           for (int m = 0; m < 8; m++) {
-            acc[m] = acc[m] + sc * (ph[m] * v[m]);
-            ph[m] = ph[m] * ap.ph2[t + T * m];
+            cplx s = mk(0, 0);
+            if (lf < 2)
+              for (int f = 0; f < FA; f++) s = s + lds[base + f * E + lpad(t + T * m)];
+            v[m] = s;
+          }
+          fft_field<L, false>(v, buf, t, tw);
+          if (lf < 2) {
+            const double sc = lf == 1 ? sO : sE;
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+              const int q = lpad(t + T * m);
+              const cplx phm = phL[q];
+              accL[q] = accL[q] + sc * (phm * v[m]);
+              phL[q] = phm * ap.ph2[t + T * m];
+            }
           }
         }
       }
@@ -327,19 +339,28 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
   }
   if (gpumpWG) emit_pump_row<L>(carry, lds, lf, t, gpumpWG);  // row 0
   if (envAcc) {
-    cplx s0[8], unused[8];
-#pragma unroll
-    for (int m = 0; m < 8; m++) s0[m] = unused[m] = mk(0, 0);
-    wg_sum_rows<L>(carry, carry, lds, lf, t, s0, unused);
-    if (lf == 0) {
-#pragma unroll
-      for (int m = 0; m < 8; m++) ap.genvT[(size_t)blockIdx.x * N + t + T * m] = s0[m];
-    }
-    if (lf < 2) {
+    // row 0 enters the envelope directly: workgroup sum of its gradient
+    if constexpr (FA == 1) {
 #pragma unroll
       for (int m = 0; m < 8; m++) {
-        ap.genvF[((size_t)blockIdx.x * 2 + lf) * N + t + T * m] = acc[m];
-        if (FA == 1) ap.genvF[((size_t)blockIdx.x * 2 + 1) * N + t + T * m] = mk(0, 0);
+        ap.genvT[(size_t)blockIdx.x * N + t + T * m] = carry[m];
+        ap.genvF[((size_t)blockIdx.x * 2) * N + t + T * m] = acc[m];
+        ap.genvF[((size_t)blockIdx.x * 2 + 1) * N + t + T * m] = mk(0, 0);
+      }
+    } else {
+      lds_barrier();
+#pragma unroll
+      for (int m = 0; m < 8; m++) lds[lf * E + lpad(t + T * m)] = carry[m];
+      lds_barrier();
+      for (int k = threadIdx.x; k < N; k += AdjLaunch<L>::THREADS) {
+        cplx s = mk(0, 0);
+        for (int f = 0; f < FA; f++) s = s + lds[f * E + lpad(k)];
+        ap.genvT[(size_t)blockIdx.x * N + k] = s;
+      }
+      if (lf < 2) {
+#pragma unroll
+        for (int m = 0; m < 8; m++)
+          ap.genvF[((size_t)blockIdx.x * 2 + lf) * N + t + T * m] = accL[lpad(t + T * m)];
       }
     }
   }

@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(Launch<L>::THREADS)
       f[m] = F0[k];
     } else {
       double s, c;
-      sincos(zi * Dp[k], &s, &c);
+      sincos_reduced(zi * Dp[k], &s, &c);
       f[m] = F0[k] * mk(c, s);
     }
     tm[m] = f[m];

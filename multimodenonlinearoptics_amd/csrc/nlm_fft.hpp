@@ -57,6 +57,30 @@ struct FftShape {
 
 __device__ __forceinline__ int lpad(int x) { return x + (x >> 3); }
 
+// sin and cos of a large argument (dispersion phases reach 1e7 rad) to about
+// one ulp: Cody-Waite reduction by pi/2 split into two doubles, each
+// subtraction done with a fused multiply-add, then the library kernel on
+// |r| <= pi/4.  Past 2^30 quarter turns the library routine is used as is.
+__device__ __forceinline__ void sincos_reduced(double a, double* s, double* c) {
+  constexpr double P1 = 1.5707963267948966;     // nearest double to pi/2
+  constexpr double P2 = 6.123233995736766e-17;  // pi/2 - P1
+  constexpr double TWO_OVER_PI = 0.6366197723675814;
+  const double k = rint(a * TWO_OVER_PI);
+  if (fabs(k) > 1073741824.0) {
+    sincos(a, s, c);
+    return;
+  }
+  const double r = fma(-k, P2, fma(-k, P1, a));
+  double sr, cr;
+  sincos(r, &sr, &cr);
+  switch (((long long)k) & 3) {
+    case 0: *s = sr; *c = cr; break;
+    case 1: *s = cr; *c = -sr; break;
+    case 2: *s = -sr; *c = -cr; break;
+    default: *s = -cr; *c = sr; break;
+  }
+}
+
 // Workgroup barrier ordering LDS only: unlike __syncthreads() it does not wait
 // for outstanding global loads, so loads issued before a transform (the next
 // z step's pump rows) stay in flight across its LDS exchanges.

@@ -46,7 +46,7 @@ def build(force=False):
         return _LIB_PATH
     import subprocess
     tmp = _LIB_PATH + ".tmp%d" % os.getpid()
-    cmd = ["gcc", "-O3", "-fno-tree-vectorize", "-march=x86-64-v3", "-fno-math-errno", "-fcx-limited-range", "-fPIC", "-shared", "-pthread",
+    cmd = ["gcc", "-O3", "-fno-tree-vectorize", "-march=x86-64-v3", "-ffp-contract=off", "-fno-math-errno", "-fcx-limited-range", "-fPIC", "-shared", "-pthread",
            "-o", tmp, src, "-lm"]
     subprocess.check_call(cmd)
     os.replace(tmp, _LIB_PATH)

@@ -125,3 +125,30 @@ def test_adjoint_errors(nlm):
                       tPrecision=64)
     with pytest.raises(ValueError):
         sfg.greensForward(False, True, 0)
+
+
+@pytest.mark.parametrize("N,nZ,ckpt", [(1024, 33, 0), (2048, 17, 4), (256, 50, 7)])
+def test_gradients_match_fd_of_device_forward(nlm, N, nZ, ckpt):
+    """Larger sizes (one and two fields per workgroup in the reverse sweep): the
+    finite differences come from the device forward, itself pinned to the oracle
+    by the parity tests."""
+    kw, env, pol = _kwargs(N, nZ, SEED0 + 7 * N + nZ)
+    r = np.random.default_rng(13)
+    Wc = r.standard_normal((N, N)) + 1j * r.standard_normal((N, N))
+    Ws = r.standard_normal((N, N)) + 1j * r.standard_normal((N, N))
+    m = nlm.Chi2PDC(customPump=env, **kw)
+    m.setPolingProfile(pol)
+    m.greensForward(False, True, ckpt)
+    g_env, g_pol = m.greensAdjoint(Wc, Ws)
+
+    def L(e, p):
+        m.setPump(e)
+        m.setPolingProfile(p)
+        return _linear_loss(*m.computeGreensFunction(), Wc, Ws)
+
+    for de, dp in ((r.standard_normal(N) + 1j * r.standard_normal(N), np.zeros(nZ)),
+                   (np.zeros(N, complex), r.standard_normal(nZ))):
+        eps = 1e-5
+        fd = (L(env + eps * de, pol + eps * dp) - L(env - eps * de, pol - eps * dp)) / (2 * eps)
+        ad = float(np.real(np.vdot(g_env, de)) + np.dot(g_pol, dp))
+        assert abs(ad - fd) <= 1e-6 * max(abs(fd), 1e-3), (ad, fd)
