@@ -21,7 +21,7 @@ typedef void (*adj_fn)(const AParams&, int grid, hipStream_t);
 
 template <int L>
 static void launch_adj(const AParams& ap, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_adjoint_pdc<L>), dim3(grid), dim3(AdjLaunch<L>::THREADS), 0, s, ap);
+  hipLaunchKernelGGL((k_adj_rev<L>), dim3(grid), dim3(AdjLaunch<L>::THREADS), 0, s, ap);
 }
 
 #define NLM_L_SWITCH2(FN, L) \
@@ -117,8 +117,8 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
   const int nWaves = nWG * (TH / 64);
   int e;
   if ((e = p->gC.ensure(total)) || (e = p->gS.ensure(total)) || (e = p->gG.ensure(nF * W)) ||
-      (e = p->seg.ensure(nF * p->ckptEvery * N)) || (e = p->gcoefPart.ensure((size_t)nWaves * nZ * 3)) ||
-      (e = p->gcoef.ensure((size_t)nZ * 3)) || (e = p->stepTab.ensure((size_t)nZ * N * 2)))
+      (e = p->gcoefPart.ensure((size_t)nWaves * nZ * 3)) || (e = p->gcoef.ensure((size_t)nZ * 3)) ||
+      (e = p->stepTab.ensure((size_t)nZ * N * 2)) || (e = p->stepInv.ensure((size_t)nZ * N * 2)))
     return e;
   if (g_pump_time) {
     if ((e = p->gpumpPart.ensure((size_t)nWG * nZP * N)) || (e = p->gpump.ensure((size_t)nZP * N))) return e;
@@ -133,8 +133,9 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
   HIP_TRY(hipMemsetAsync(p->gcoefPart.p, 0, (size_t)nWaves * nZ * 3 * sizeof(cplx), p->stream));
   hipLaunchKernelGGL(k_greens_assemble_adjoint, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, p->stream,
                      p->gC.p, p->gS.p, p->gG.p, N, 1, 1, 1, (const cplx*)nullptr);
-  // the recomputation between checkpoints uses the step propagators
-  nlm_launch_step_table(NLM_CHI2PDC, p->coef.p, p->cconst.p, p->pumpT[0].p, N, nZ, p->stepTab.p, p->stream);
+  // the states between checkpoints are rebuilt with the inverse step propagators
+  nlm_launch_step_table(NLM_CHI2PDC, p->coef.p, p->cconst.p, p->pumpT[0].p, N, nZ, p->stepTab.p, p->stream,
+                        p->stepInv.p);
   HIP_TRY(hipGetLastError());
   AParams ap;
   memset(&ap, 0, sizeof ap);
@@ -148,7 +149,8 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
   ap.outPos[0] = 0;
   ap.outPos[1] = ap.outPos[2] = ap.outPos[3] = -1;
   ap.ckpt = p->ckpt.p;
-  ap.seg = p->seg.p;
+  ap.fout = p->G.p;
+  ap.inv = p->stepInv.p;
   ap.gout = p->gG.p;
   ap.gin = nullptr;
   ap.gcoef = p->gcoefPart.p;

@@ -8,10 +8,10 @@
 // g_a = FFT(g_b) / N.
 //
 // The forward Green's launch (k_prop_tab with kp.ckpt set) stores the
-// time-domain field every K steps.  For each segment, in reverse order,
-// k_adjoint_pdc recomputes the segment's states from its checkpoint with the
-// same step propagators as the forward pass, then sweeps the segment
-// backwards: adjoint of the dispersion step (forward FFT, conj(exp(i dz D)),
+// time-domain field every K steps.  k_adj_rev sweeps all steps backwards,
+// rebuilding each forward state on chip by inverting the step (the states
+// between checkpoints are recomputed, backwards, instead of replayed from the
+// checkpoint and stored): adjoint of the dispersion step (forward FFT, conj(exp(i dz D)),
 // inverse FFT), then the transposed RK4 Jacobian of the DiffEq.  The per-step
 // coefficient gradients are reduced within each wave (deterministic shuffles)
 // and written as per-wave partials.  The pump enters through its linear
@@ -33,7 +33,8 @@ struct AParams {
   int inTime, outTime, nOut;
   int outPos[4];
   const cplx* ckpt;    // [nFields][nSeg][N] scaled time-domain states a_{sK}
-  cplx* seg;           // [nFields][K][N] recomputed states of the current segment
+  const cplx* fout;    // [nFields][nOut*N] the forward's output rows, domain = outTime
+  const cplx* inv;     // [nZ][N][2] inverse RK4 step propagators
   const cplx* gout;    // [nFields][nOut*N], domain = outTime
   cplx* gin;           // [nFields][N] gradient w.r.t. the input (input domain) or nullptr
   cplx* gcoef;         // [nWaves][nZ][3] per-wave partials of the step coefficients
@@ -115,9 +116,14 @@ NLM_DEV void emit_pump_row(const cplx (&v)[8], cplx* lds, int lf, int t, cplx* d
 }
 
 // Reverse sweep of the Chi2PDC Green's function (k = c P conj(y), one signal
-// mode, one pump): one field per T lanes, FA fields per workgroup.
+// mode, one pump): one field per T lanes, FA fields per workgroup, all steps in
+// one launch.  The forward states are rebuilt backwards on chip: the
+// dispersion step is unitary and the RK4 step x -> alpha x + beta conj(x) of
+// this medium is invertible, so a_{i-1} = R_i^-1(IFFT(FFT(a_i) conj(exp(i dz D)))),
+// starting from the forward's output row and replaced by the stored
+// checkpoint at every multiple of K steps, which bounds the rounding drift.
 template <int L>
-__global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams ap) {
+__global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adj_rev(AParams ap) {
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T, FA = AdjLaunch<L>::FA;
   __shared__ cplx lds[AdjLaunch<L>::LDS_ELEMS];
@@ -132,9 +138,11 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
 
   FftTwiddles<L> tw;
   fft_load_twiddles<L>(tw, ap.tw, t);
+  const int i1 = ap.nZ - 1;
 
   // gradient of the last frequency row: output = F_last * halfOut (freq) or its IFFT (time)
   cplx g[8];
+  cplx carry[8];  // pump row 2i gradient from step i+1 (its k1 stage)
   const int op = ap.outPos[0];
 #pragma unroll
   for (int m = 0; m < 8; m++)
@@ -144,12 +152,35 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
     scale_inv<L>(g);
   }
 #pragma unroll
-  for (int m = 0; m < 8; m++) g[m] = conj(ap.halfOut[t + T * m]) * g[m];
+  for (int m = 0; m < 8; m++) {
+    g[m] = conj(ap.halfOut[t + T * m]) * g[m];
+    carry[m] = mk(0, 0);
+  }
   bool gFreq = true;  // g currently holds a frequency-domain gradient
 
-  cplx carry[8];  // pump row 2i gradient from step i+1 (its k1 stage)
+  // state a_{i-1} of the step being reversed, time domain
+  cplx x[8];
+  if (i1 >= 1) {
+    if ((i1 - 1) % ap.K == 0) {
 #pragma unroll
-  for (int m = 0; m < 8; m++) carry[m] = mk(0, 0);
+      for (int m = 0; m < 8; m++) x[m] = ap.ckpt[((size_t)fld * ap.nSeg + (i1 - 1) / ap.K) * N + t + T * m];
+    } else {
+      // F_last = output * conj(halfOut), r_{nZ-1} = IFFT(F_last conj(exp(i dz D))), a = R^-1(r)
+#pragma unroll
+      for (int m = 0; m < 8; m++) x[m] = ap.fout[(size_t)fld * ap.nOut * N + (size_t)op * N + t + T * m];
+      if (ap.outTime) fft_field<L, false>(x, buf, t, tw);
+#pragma unroll
+      for (int m = 0; m < 8; m++) x[m] = conj(ap.halfOut[t + T * m] * ap.dstep[t + T * m]) * x[m];
+      fft_field<L, true>(x, buf, t, tw);
+      scale_inv<L>(x);
+      const cplx* iv = ap.inv + (size_t)i1 * N * 2;
+#pragma unroll
+      for (int m = 0; m < 8; m++) {
+        const cplx ia = iv[(size_t)(t + T * m) * 2], ib = iv[(size_t)(t + T * m) * 2 + 1];
+        x[m] = ia * x[m] + ib * conj(x[m]);
+      }
+    }
+  }
   cplx* gpumpWG = ap.gpump ? ap.gpump + (size_t)blockIdx.x * nZP * N : nullptr;
 
   // envelope gradient accumulators: group 0 takes the even pump rows 2i,
@@ -162,7 +193,7 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
   cplx* accL = lds + AdjLaunch<L>::ACC_OFF + (lf & 1) * 2 * E;
   cplx* phL = accL + E;
   if (envAcc && lf < 2) {
-    const int r0 = lf == 0 ? 2 * (ap.nZ - 1) : 2 * (ap.nZ - 1) - 1;
+    const int r0 = lf == 0 ? 2 * i1 : 2 * i1 - 1;
 #pragma unroll
     for (int m = 0; m < 8; m++) {
       double sn, cs;
@@ -177,23 +208,8 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
     }
   }
 
-  cplx* segF = ap.seg + (size_t)fld * ap.K * N;
-  for (int s = ap.nSeg - 1; s >= 0; s--) {
-    const int i0 = s * ap.K;
-    const int i1 = min(i0 + ap.K, ap.nZ - 1);
-    {
-      cplx a[8];
-#pragma unroll
-      for (int m = 0; m < 8; m++) a[m] = ap.ckpt[((size_t)fld * ap.nSeg + s) * N + t + T * m];
-      for (int i = i0 + 1; i <= i1; i++) {
-        if (active) {
-#pragma unroll
-          for (int m = 0; m < 8; m++) segF[(size_t)(i - i0 - 1) * N + t + T * m] = a[m];
-        }
-        if (i < i1) fwd_step_tab<L>(a, i, ap, buf, t, tw);
-      }
-    }
-    for (int i = i1; i > i0; i--) {
+  {
+    for (int i = i1; i >= 1; i--) {
       // adjoint of the dispersion step: F = FFT(r) exp(i dz D); a_i = IFFT(F) (i < nZ-1)
       if (!gFreq) {
         fft_field<L, false>(g, buf, t, tw);
@@ -220,14 +236,14 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
 #pragma unroll
       for (int m = 0; m < 8; m++) {
         const int k = t + T * m;
-        const cplx x = active ? segF[(size_t)(i - i0 - 1) * N + k] : mk(0, 0);
+        const cplx xs = x[m];
         const cplx P0 = ap.pump0[(size_t)(2 * i - 2) * N + k];
         const cplx P1 = ap.pump0[(size_t)(2 * i - 1) * N + k];
         const cplx P2 = ap.pump0[(size_t)(2 * i) * N + k];
         const cplx m0 = c0 * P0, m1 = c1 * P1, m2 = c2 * P2;
-        const cplx y2 = x + 0.5 * (m0 * conj(x));
-        const cplx y3 = x + 0.5 * (m1 * conj(y2));
-        const cplx y4 = x + m1 * conj(y3);
+        const cplx y2 = xs + 0.5 * (m0 * conj(xs));
+        const cplx y3 = xs + 0.5 * (m1 * conj(y2));
+        const cplx y4 = xs + m1 * conj(y3);
         const cplx gr = g[m];
         const cplx gk4 = gr * (1. / 6.);
         cplx gk3 = gr * (1. / 3.), gk2 = gr * (1. / 3.), gk1 = gk4, gx = gr, gy;
@@ -244,7 +260,7 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
         gx = gx + gy;
         gk1 = gk1 + 0.5 * gy;
         gy = m0 * conj(gk1);
-        const cplx gm0 = x * gk1;
+        const cplx gm0 = xs * gk1;
         gx = gx + gy;
         g[m] = active ? gx : mk(0, 0);
         gc0 = gc0 + conj(P0) * gm0;
@@ -332,6 +348,25 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adjoint_pdc(AParams a
               accL[q] = accL[q] + sc * (phm * v[m]);
               phL[q] = phm * ap.ph2[t + T * m];
             }
+          }
+        }
+      }
+      // state a_{i-2} for the next (earlier) step
+      if (i >= 2) {
+        if ((i - 2) % ap.K == 0) {
+#pragma unroll
+          for (int m = 0; m < 8; m++) x[m] = ap.ckpt[((size_t)fld * ap.nSeg + (i - 2) / ap.K) * N + t + T * m];
+        } else {
+          fft_field<L, false>(x, buf, t, tw);
+#pragma unroll
+          for (int m = 0; m < 8; m++) x[m] = conj(ap.dstep[t + T * m]) * x[m];
+          fft_field<L, true>(x, buf, t, tw);
+          scale_inv<L>(x);
+          const cplx* iv = ap.inv + (size_t)(i - 1) * N * 2;
+#pragma unroll
+          for (int m = 0; m < 8; m++) {
+            const cplx ia = iv[(size_t)(t + T * m) * 2], ib = iv[(size_t)(t + T * m) * 2 + 1];
+            x[m] = ia * x[m] + ib * conj(x[m]);
           }
         }
       }

@@ -427,8 +427,8 @@ extern "C" int nlm_plan_destroy(nlm_plan* p) {
                           &p->pumpT[0], &p->pumpT[1], &p->pumpF[0], &p->pumpF[1], &p->gridF, &p->gridT,
                           &p->G, &p->C, &p->S, &p->in, &p->out, &p->env, &p->F0, &p->dsp, &p->relI};
   for (auto* b : bufs) b->release();
-  DevBuf<cplx>* abufs[] = {&p->ckpt, &p->seg, &p->gG, &p->gC, &p->gS, &p->gcoefPart, &p->gcoef, &p->gpumpPart,
-                           &p->gpump, &p->gF0Part, &p->gF0, &p->gtmp, &p->scale};
+  DevBuf<cplx>* abufs[] = {&p->ckpt, &p->gG, &p->gC, &p->gS, &p->gcoefPart, &p->gcoef, &p->gpumpPart, &p->gpump,
+                           &p->gF0Part, &p->gF0, &p->gtmp, &p->scale, &p->stepTab, &p->stepInv};
   for (auto* b : abufs) b->release();
   p->Dp.release();
   p->strength.release();

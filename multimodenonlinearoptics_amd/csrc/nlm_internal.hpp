@@ -24,6 +24,10 @@ template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
   int ensure(size_t count) {
     if (count <= n && p) return NLM_OK;
     if (p) (void)hipFree(p);
@@ -59,8 +63,9 @@ struct nlm_plan_s {
   // Green's forward kept for the adjoint
   bool pumpLinear = false, hasStrength = false;
   double dzp = 0;
-  DevBuf<cplx> ckpt, seg, gG, gC, gS, gcoefPart, gcoef, gpumpPart, gpump, gF0Part, gF0, gtmp;
+  DevBuf<cplx> ckpt, gG, gC, gS, gcoefPart, gcoef, gpumpPart, gpump, gF0Part, gF0, gtmp;
   DevBuf<cplx> scale;
+  DevBuf<cplx> stepInv;  // inverse RK4 step propagators (adjoint)
   DevBuf<cplx> stepTab;  // RK4 step propagators of one-mode undepleted media [nZ][N][2]
   int stepTable = 1;      // use them (nlm_plan_set_step_table)
   int ckptEvery = 0, nSeg = 0, gNIn = 0, gNOut = 0, gInTime = 0, gHasScale = 0;
