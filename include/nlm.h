@@ -141,6 +141,38 @@ int nlm_batch_dev(nlm_plan* plan, const double* inputs_dev, int rows, int cols, 
 
 /* Per-launch device timing of the last call (ms), recorded with HIP events on
  * the plan's stream when enabled: [0] pump, [1] propagation, [2] assembly. */
+/* Heterogeneous batches: n independent propagations of different media (model,
+ * nFreqs, nZSteps, dispersion, pump), each the batchSignalSimulation of one
+ * input row (src/_NonlinearMedium.cpp:378-444, src/_FullyNonlinearMedium.hpp:8-11),
+ * run as one captured graph of per-(model, nFreqs) persistent kernels fed by
+ * device work queues.  Supported: Chi2PDC (pump rows generated in the kernel
+ * from the envelope, as in src/_NonlinearMedium.cpp:225-250), Chi2SHG, Chi2ASHG;
+ * nFreqs 32..4096.  The plans must outlive the batch.  (Extension: no
+ * counterpart in the reference, which runs instances one by one.) */
+typedef struct {
+  nlm_plan* plan;
+  const double* input;       /* cols complex values; copied to the device at creation */
+  int32_t cols;
+  int32_t in_time_domain;
+  int32_t input_mode;
+  int32_t n_out;
+  int32_t outputs[4];
+  const double* envelope;    /* linear pump: [N] complex time-domain envelope, else NULL */
+  const double* disp_pump;   /* [N] real D_p(omega) */
+  double dzp;
+  const double* strength;    /* [2nZ-1] beam-profile factor or NULL */
+} nlm_multi_item;
+typedef struct nlm_multi_s nlm_multi;
+/* autotune: time each class alone for a few persistent grid sizes and keep the
+ * fastest (once per process and class shape). */
+int nlm_multi_create(int n, const nlm_multi_item* items, int autotune, nlm_multi** out);
+int nlm_multi_launch(nlm_multi* batch);             /* one graph launch, asynchronous */
+int nlm_multi_sync(nlm_multi* batch, double* ms);   /* wait; ms = device time of the last launch */
+int nlm_multi_output(nlm_multi* batch, int i, double* out);  /* n_out*N complex of instance i */
+int nlm_multi_classes(nlm_multi* batch);
+int nlm_multi_grid(nlm_multi* batch, int c, int32_t* model, int32_t* log2n, int32_t* count, int32_t* grid);
+int nlm_multi_destroy(nlm_multi* batch);
+
 /* Single-signal-mode undepleted media (Chi3, Chi2PDC): propagate with the
  * per-step RK4 propagator table (enable = 1, the default) or evaluate the
  * DiffEq in every field (enable = 0).  Both compute the RK4 step of

@@ -585,7 +585,7 @@ int check_pumps_ready(nlm_plan* p) {
 }
 
 // Input channel of each mode (src/_NonlinearMedium.hpp:164-198).
-static void input_channels(int nS, int nCh, int inputMode, int* inPos) {
+void input_channels(int nS, int nCh, int inputMode, int* inPos) {
   if (nCh > 1) inputMode = 0;
   if (nS <= 1) inputMode = 0;
   for (int q = 0; q < nS; q++) {

@@ -1263,4 +1263,77 @@ Grid Cascade::batchSignalSimulation(const Grid& inputProfs, bool inTimeDomain, b
   return out;
 }
 
+// ---------------------------------------------------------------------------
+// MultiBatch
+// ---------------------------------------------------------------------------
+
+MultiBatch::MultiBatch(const std::vector<Medium*>& media, const std::vector<std::vector<cd>>& inputs,
+                       bool inTimeDomain, bool autotune)
+    : media_(media) {
+  if (media.size() != inputs.size()) throw std::invalid_argument("one input row per medium is required");
+  const size_t n = media.size();
+  std::vector<nlm_multi_item> items(n);
+  std::vector<std::vector<double>> strengths(n);
+  for (size_t i = 0; i < n; i++) {
+    Medium* md = media[i];
+    if (!md) throw std::invalid_argument("null medium");
+    nlm_multi_item& it = items[i];
+    std::memset(&it, 0, sizeof it);
+    it.plan = md->plan();
+    it.input = reinterpret_cast<const double*>(inputs[i].data());
+    it.cols = (int32_t)inputs[i].size();
+    it.in_time_domain = inTimeDomain ? 1 : 0;
+    it.input_mode = 0;
+    it.n_out = md->nS_;
+    for (int q = 0; q < md->nS_ && q < 4; q++) it.outputs[q] = q;
+    if (md->nP_ == 1 && md->model_ == NLM_CHI2PDC) {
+      it.envelope = reinterpret_cast<const double*>(md->envelope_[0].data());
+      it.disp_pump = md->dispPump_[0].data();
+      it.dzp = md->dzp_;
+      if (md->profile_ != IntensityProfile::Constant) {
+        const std::vector<double> x = md->linSpaced(md->nZP_, -0.5 * md->z_, 0.5 * md->z_);
+        strengths[i].resize(md->nZP_);
+        for (int r = 0; r < md->nZP_; r++) {
+          const double u = x[r] / md->zR_;
+          strengths[i][r] = md->profile_ == IntensityProfile::GaussianApodization ? std::exp(-0.5 * (u * u))
+                                                                                  : 1 / std::sqrt(1 + u * u);
+        }
+        it.strength = strengths[i].data();
+      }
+    }
+    outLen_.push_back(md->nS_ * md->N_);
+  }
+  check(nlm_multi_create((int)n, items.data(), autotune ? 1 : 0, &m_));
+}
+
+MultiBatch::~MultiBatch() {
+  if (m_) nlm_multi_destroy(m_);
+}
+
+void MultiBatch::launch() { check(nlm_multi_launch(m_)); }
+
+double MultiBatch::sync() {
+  double ms = 0;
+  check(nlm_multi_sync(m_, &ms));
+  return ms;
+}
+
+std::vector<cd> MultiBatch::output(int i) const {
+  if (i < 0 || i >= (int)media_.size()) throw std::out_of_range("instance index out of range");
+  std::vector<cd> out(outLen_[i]);
+  check(nlm_multi_output(m_, i, reinterpret_cast<double*>(out.data())));
+  return out;
+}
+
+std::vector<std::array<int, 4>> MultiBatch::classes() const {
+  std::vector<std::array<int, 4>> r;
+  const int nc = nlm_multi_classes(m_);
+  for (int c = 0; c < nc; c++) {
+    int32_t model, l, count, grid;
+    check(nlm_multi_grid(m_, c, &model, &l, &count, &grid));
+    r.push_back({model, l, count, grid});
+  }
+  return r;
+}
+
 }  // namespace nlm

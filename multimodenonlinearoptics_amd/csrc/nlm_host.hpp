@@ -10,6 +10,7 @@
 #pragma once
 #include <complex>
 #include <cstdint>
+#include <array>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -48,6 +49,7 @@ void check(int status);
 
 class Medium {
   friend class Cascade;
+  friend class MultiBatch;
 
  public:
   virtual ~Medium();
@@ -247,6 +249,34 @@ class Chi3GNLSE : public FullyNonlinearMedium {
 
  protected:
   void stepCoefficients(std::vector<cd>& coef, std::vector<cd>& cconst) const override;
+};
+
+// Heterogeneous batch (extension): one input row for each of many media of
+// different models, sizes and parameters, propagated together by one graph
+// launch of per-(model, nFreqs) persistent kernels (nlm_multi_* in nlm.h).
+// Each output equals media[i].batchSignalSimulation(inputs[i:i+1], inTimeDomain,
+// runPump=True) of that medium (src/_NonlinearMedium.cpp:378-444).
+class MultiBatch {
+ public:
+  MultiBatch(const std::vector<Medium*>& media, const std::vector<std::vector<cd>>& inputs, bool inTimeDomain,
+             bool autotune);
+  ~MultiBatch();
+  MultiBatch(const MultiBatch&) = delete;
+  MultiBatch& operator=(const MultiBatch&) = delete;
+  void launch();
+  double sync();  // waits; device milliseconds of the last launch
+  double run() {
+    launch();
+    return sync();
+  }
+  std::vector<cd> output(int i) const;
+  int size() const { return (int)media_.size(); }
+  std::vector<std::array<int, 4>> classes() const;  // (model, log2 N, instances, persistent workgroups)
+
+ private:
+  std::vector<Medium*> media_;
+  std::vector<int> outLen_;
+  nlm_multi* m_ = nullptr;
 };
 
 class Cascade : public Medium {

@@ -75,6 +75,8 @@ void fill_common(nlm_plan_s* p, KParams& kp);
 int check_pumps_ready(nlm_plan_s* p);
 int launch_prop_kernel(nlm_plan_s* p, const KParams& kp);
 int get_twiddles(int n, const cplx** out);
+// input channel feeding each mode (-1: none), src/_NonlinearMedium.hpp:164-198
+void input_channels(int nS, int nCh, int inputMode, int* inPos);
 void tic(nlm_plan_s* p, int slot);
 void collect_timing(nlm_plan_s* p, int a0, int a1, int slot);
 void launch_greens_assemble(const cplx* G, cplx* C, cplx* S, int N, int nIn, int nOut, int nb, const cplx* scale,

@@ -1,0 +1,422 @@
+// nlm_multi.hip -- heterogeneous batches: many independent propagations of
+// different media (model, nFreqs, nZSteps, dispersion, pump) in one graph launch.
+//
+// Each instance is one batchSignalSimulation row of its own medium
+// (src/_NonlinearMedium.cpp:378-444; the FNLM overload
+// src/_FullyNonlinearMedium.hpp:8-11).  Instances are grouped into classes by
+// (model, log2 N); every class runs as one persistent kernel whose workgroups
+// take instances from a device work queue (one atomic counter per class) in
+// order of decreasing cost nZ N log2 N, so long propagations start first and
+// short ones fill the tail.  For linear-pump media the pump rows are generated
+// inside the kernel (InlinePump), so no (2nZ-1) x N pump grid is written or
+// read per instance.  All classes are captured into one hipGraph (one branch
+// per class on its own stream, joined at the end); nlm_multi_launch replays it.
+//
+// The number of persistent workgroups of a class is chosen by a small
+// autotuner at creation: each class is timed alone for a few grid sizes and
+// the fastest kept (per process, keyed by model, log2 N and instance count).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "nlm_internal.hpp"
+#include "nlm_prop.hpp"
+
+struct MultiInst {
+  const cplx* dstep;
+  const cplx* halfIn;
+  const cplx* halfOut;
+  const cplx* coef;
+  const cplx* cconst;
+  const cplx* in;
+  cplx* out;
+  const cplx* env;
+  const double* Dp;
+  const double* strength;
+  double dzp;
+  int nZ, inTime, inCols, nOut;
+  int inPos[4], outPos[4];
+};
+
+template <int L>
+struct MultiLaunch {
+  static constexpr int THREADS = FftShape<L>::T;  // one instance per workgroup
+};
+
+template <class M, int L, bool INLINE>
+__global__ void __launch_bounds__(MultiLaunch<L>::THREADS)
+    k_prop_multi(const MultiInst* __restrict__ inst, const int* __restrict__ order, int n, int* counter,
+                 const cplx* tw) {
+  using F = FftShape<L>;
+  __shared__ cplx lds[F::lds_elems];
+  __shared__ int item;
+  const int t = threadIdx.x;
+  for (;;) {
+    __syncthreads();  // the previous instance's readers of `item` are done
+    if (t == 0) item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(item);
+    if (w >= n) break;  // uniform: every workgroup leaves once the queue is empty
+    const MultiInst& I = inst[order[w]];
+    KParams kp{};
+    kp.nZ = I.nZ;
+    kp.kind = KIND_BATCH;
+    kp.nFields = 1;
+    kp.inTime = I.inTime;
+    kp.outTime = I.inTime;
+    kp.inCols = I.inCols;
+    kp.nOut = I.nOut;
+    for (int q = 0; q < 4; q++) {
+      kp.inPos[q] = I.inPos[q];
+      kp.outPos[q] = I.outPos[q];
+    }
+    kp.in = I.in;
+    kp.out = I.out;
+    kp.tw = tw;
+    kp.dstep = I.dstep;
+    kp.halfIn = I.halfIn;
+    kp.halfOut = I.halfOut;
+    kp.coef = I.coef;
+    kp.cconst = I.cconst;
+    kp.env = I.env;
+    kp.Dp = I.Dp;
+    kp.strength = I.strength;
+    kp.dzp = I.dzp;
+    prop_field<M, L, INLINE>(kp, 0, true, t, lds);
+  }
+}
+
+__global__ void k_zero_ints(int* p, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0;
+}
+
+typedef void (*multi_fn)(const MultiInst*, const int*, int, int*, const cplx*, int grid, hipStream_t);
+
+template <class M, int L, bool INLINE>
+static void launch_multi(const MultiInst* inst, const int* order, int n, int* counter, const cplx* tw, int grid,
+                         hipStream_t s) {
+  hipLaunchKernelGGL((k_prop_multi<M, L, INLINE>), dim3(grid), dim3(MultiLaunch<L>::THREADS), 0, s, inst, order, n,
+                     counter, tw);
+}
+
+#define NLM_MULTI_SWITCH(M, INL)                   \
+  switch (L) {                                     \
+    case 5: return &launch_multi<M, 5, INL>;       \
+    case 6: return &launch_multi<M, 6, INL>;       \
+    case 7: return &launch_multi<M, 7, INL>;       \
+    case 8: return &launch_multi<M, 8, INL>;       \
+    case 9: return &launch_multi<M, 9, INL>;       \
+    case 10: return &launch_multi<M, 10, INL>;     \
+    case 11: return &launch_multi<M, 11, INL>;     \
+    case 12: return &launch_multi<M, 12, INL>;     \
+  }                                                \
+  return nullptr;
+
+// Models with a heterogeneous-batch kernel: Chi2PDC (pump rows generated in
+// the kernel) and the second-harmonic media Chi2SHG / Chi2ASHG (no pump).
+static multi_fn pick_multi(int model, int L) {
+  if (model == NLM_CHI2PDC) {
+    NLM_MULTI_SWITCH(MChi2PDC, true)
+  }
+  if (model == NLM_CHI2SHG || model == NLM_CHI2ASHG) {
+    NLM_MULTI_SWITCH(MChi2SHG, false)
+  }
+  return nullptr;
+}
+
+struct MultiClass {
+  int model, L, count, grid;
+  multi_fn fn;
+  size_t orderOff;  // offset of this class's instance order in the order array
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+};
+
+struct nlm_multi_s {
+  int n = 0;
+  std::vector<int> N, nOut;
+  std::vector<size_t> outOff;
+  DevBuf<cplx> inBuf, outBuf, envBuf;
+  DevBuf<double> realBuf;
+  DevBuf<MultiInst> inst;
+  DevBuf<int> order, counters;
+  std::vector<MultiClass> classes;
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  float lastMs = 0.f;
+};
+
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int>, int> g_tuned;  // (model, L, count) -> grid
+
+static int device_cus() {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return cus > 0 ? cus : 256;
+}
+
+static int run_class(nlm_multi_s* m, const MultiClass& c, int grid, const cplx* tw, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, m->counters.p + (&c - m->classes.data()), 1);
+  c.fn(m->inst.p, m->order.p + c.orderOff, c.count, m->counters.p + (&c - m->classes.data()), tw, grid, s);
+  HIP_TRY(hipGetLastError());
+  return NLM_OK;
+}
+
+extern "C" int nlm_multi_destroy(nlm_multi* m) {
+  if (!m) return NLM_OK;
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  if (m->exec) (void)hipGraphExecDestroy(m->exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  for (auto& c : m->classes) {
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    if (c.done) (void)hipEventDestroy(c.done);
+  }
+  if (m->fork) (void)hipEventDestroy(m->fork);
+  if (m->t0) (void)hipEventDestroy(m->t0);
+  if (m->t1) (void)hipEventDestroy(m->t1);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return NLM_OK;
+}
+
+extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune, nlm_multi** out) {
+  if (!out || n < 0 || (n > 0 && !items)) return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
+  *out = nullptr;
+  nlm_multi_s* m = new nlm_multi_s();
+  auto fail = [&](int code) {
+    nlm_multi_destroy(m);
+    return code;
+  };
+  m->n = n;
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&m->fork) != hipSuccess || hipEventCreate(&m->t0) != hipSuccess ||
+      hipEventCreate(&m->t1) != hipSuccess)
+    return fail(set_err(NLM_DEVICE_ERROR, "stream/event creation failed"));
+
+  // sizes of the packed per-instance buffers
+  size_t inTot = 0, outTot = 0, envTot = 0, realTot = 0;
+  std::map<std::pair<int, int>, std::vector<int>> byClass;
+  m->N.resize(n);
+  m->nOut.resize(n);
+  m->outOff.resize(n);
+  for (int i = 0; i < n; i++) {
+    const nlm_multi_item& it = items[i];
+    nlm_plan* p = it.plan;
+    if (!p) return fail(set_err(NLM_INVALID_ARGUMENT, "null plan in item"));
+    if (!pick_multi(p->model, p->L))
+      return fail(set_err(NLM_INVALID_ARGUMENT,
+                          "heterogeneous batches support Chi2PDC, Chi2SHG and Chi2ASHG with nFreqs 32..4096"));
+    if (!it.input || it.cols <= 0 || it.cols % p->N != 0 || it.cols / p->N > p->nS)
+      return fail(set_err(NLM_INVALID_ARGUMENT, "Signals not of correct length!"));
+    if (it.input_mode < 0 || it.input_mode >= p->nS)
+      return fail(set_err(NLM_INVALID_ARGUMENT, "inputModes does not match any mode in the system"));
+    if (it.n_out < 1 || it.n_out > p->nS) return fail(set_err(NLM_INVALID_ARGUMENT, "Requested no outputs!"));
+    for (int q = 0; q < it.n_out; q++)
+      if (it.outputs[q] < 0 || it.outputs[q] >= p->nS)
+        return fail(set_err(NLM_INVALID_ARGUMENT, "output mode out of range"));
+    const bool pumped = p->model == NLM_CHI2PDC;
+    if (pumped && (!it.envelope || !it.disp_pump))
+      return fail(set_err(NLM_INVALID_ARGUMENT, "a pumped medium needs its pump envelope and dispersion"));
+    m->N[i] = p->N;
+    m->nOut[i] = it.n_out;
+    m->outOff[i] = outTot;
+    inTot += it.cols;
+    outTot += (size_t)it.n_out * p->N;
+    if (pumped) {
+      envTot += p->N;
+      realTot += p->N + (it.strength ? 2 * (size_t)p->nZ - 1 : 0);
+    }
+    byClass[{p->model == NLM_CHI2ASHG ? NLM_CHI2SHG : p->model, p->L}].push_back(i);
+  }
+  int e;
+  if ((e = m->inBuf.ensure(inTot)) || (e = m->outBuf.ensure(outTot)) || (e = m->envBuf.ensure(envTot)) ||
+      (e = m->realBuf.ensure(realTot)) || (e = m->inst.ensure(n)) || (e = m->order.ensure(n)) ||
+      (e = m->counters.ensure(byClass.size() + 1)))
+    return fail(e);
+
+  // descriptors and packed uploads
+  std::vector<MultiInst> hi(n);
+  std::vector<cplx> hin(inTot), henv(envTot);
+  std::vector<double> hreal(realTot);
+  size_t io = 0, eo = 0, ro = 0;
+  for (int i = 0; i < n; i++) {
+    const nlm_multi_item& it = items[i];
+    nlm_plan* p = it.plan;
+    MultiInst& d = hi[i];
+    memset(&d, 0, sizeof d);
+    d.dstep = p->dstep.p;
+    d.halfIn = p->halfIn.p;
+    d.halfOut = p->halfOut.p;
+    d.coef = p->coef.p;
+    d.cconst = p->cconst.p;
+    std::memcpy(hin.data() + io, it.input, sizeof(cplx) * it.cols);
+    d.in = m->inBuf.p + io;
+    io += it.cols;
+    d.out = m->outBuf.p + m->outOff[i];
+    d.nZ = p->nZ;
+    d.inTime = it.in_time_domain ? 1 : 0;
+    d.inCols = it.cols;
+    d.nOut = it.n_out;
+    for (int q = 0; q < 4; q++) d.inPos[q] = d.outPos[q] = -1;
+    input_channels(p->nS, it.cols / p->N, it.input_mode, d.inPos);
+    for (int q = 0; q < it.n_out; q++) d.outPos[it.outputs[q]] = q;
+    if (p->model == NLM_CHI2PDC) {
+      std::memcpy(henv.data() + eo, it.envelope, sizeof(cplx) * p->N);
+      d.env = m->envBuf.p + eo;
+      eo += p->N;
+      std::memcpy(hreal.data() + ro, it.disp_pump, sizeof(double) * p->N);
+      d.Dp = m->realBuf.p + ro;
+      ro += p->N;
+      if (it.strength) {
+        std::memcpy(hreal.data() + ro, it.strength, sizeof(double) * (2 * (size_t)p->nZ - 1));
+        d.strength = m->realBuf.p + ro;
+        ro += 2 * (size_t)p->nZ - 1;
+      }
+      d.dzp = it.dzp;
+    }
+  }
+  // classes and their longest-first orders
+  std::vector<int> hord;
+  hord.reserve(n);
+  for (auto& kv : byClass) {
+    MultiClass c;
+    c.model = kv.first.first;
+    c.L = kv.first.second;
+    c.fn = pick_multi(c.model, c.L);
+    std::vector<int> idx = kv.second;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return hi[a].nZ > hi[b].nZ; });
+    c.orderOff = hord.size();
+    hord.insert(hord.end(), idx.begin(), idx.end());
+    c.count = (int)idx.size();
+    c.grid = 1;
+    m->classes.push_back(c);
+  }
+  if (n > 0) {
+    HIP_TRY(hipMemcpy(m->inst.p, hi.data(), sizeof(MultiInst) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->order.p, hord.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->inBuf.p, hin.data(), sizeof(cplx) * inTot, hipMemcpyHostToDevice));
+    if (envTot) HIP_TRY(hipMemcpy(m->envBuf.p, henv.data(), sizeof(cplx) * envTot, hipMemcpyHostToDevice));
+    if (realTot) HIP_TRY(hipMemcpy(m->realBuf.p, hreal.data(), sizeof(double) * realTot, hipMemcpyHostToDevice));
+  }
+
+  // grid sizes: autotuned per class (timed alone) or the default of four
+  // workgroups per CU, never more workgroups than instances
+  const int cus = device_cus();
+  for (auto& c : m->classes) {
+    const cplx* tw = nullptr;
+    if ((e = get_twiddles(1 << c.L, &tw))) return fail(e);
+    const int cand[4] = {cus, 2 * cus, 4 * cus, 8 * cus};
+    int best = std::min(c.count, 4 * cus);
+    const auto key = std::make_tuple(c.model, c.L, c.count);
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto f = g_tuned.find(key);
+      if (f != g_tuned.end()) {
+        best = f->second;
+        cached = true;
+      }
+    }
+    if (autotune && !cached && c.count > cus) {
+      float bestMs = 1e30f;
+      for (int g : cand) {
+        const int grid = std::min(c.count, g);
+        if ((e = run_class(m, c, grid, tw, m->stream))) return fail(e);  // warm
+        HIP_TRY(hipEventRecord(m->t0, m->stream));
+        if ((e = run_class(m, c, grid, tw, m->stream))) return fail(e);
+        HIP_TRY(hipEventRecord(m->t1, m->stream));
+        HIP_TRY(hipEventSynchronize(m->t1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, m->t0, m->t1));
+        if (ms < bestMs) {
+          bestMs = ms;
+          best = grid;
+        }
+        if (grid == c.count) break;
+      }
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tuned[key] = best;
+    }
+    c.grid = std::max(1, best);
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c.done, hipEventDisableTiming) != hipSuccess)
+      return fail(set_err(NLM_DEVICE_ERROR, "stream/event creation failed"));
+  }
+
+  // capture: fork one branch per class, join on the main stream
+  if (!m->classes.empty()) {
+    HIP_TRY(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = hipEventRecord(m->fork, m->stream);
+    for (auto& c : m->classes) {
+      const cplx* tw = nullptr;
+      (void)get_twiddles(1 << c.L, &tw);
+      if (ce == hipSuccess) ce = hipStreamWaitEvent(c.stream, m->fork, 0);
+      if (ce == hipSuccess) {
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c.stream, m->counters.p + (&c - m->classes.data()), 1);
+        c.fn(m->inst.p, m->order.p + c.orderOff, c.count, m->counters.p + (&c - m->classes.data()), tw, c.grid,
+             c.stream);
+        ce = hipGetLastError();
+      }
+      if (ce == hipSuccess) ce = hipEventRecord(c.done, c.stream);
+      if (ce == hipSuccess) ce = hipStreamWaitEvent(m->stream, c.done, 0);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(m->stream, &g);
+    if (ce != hipSuccess || ee != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(set_err(NLM_DEVICE_ERROR, std::string("graph capture failed: ") +
+                                                hipGetErrorString(ce != hipSuccess ? ce : ee)));
+    }
+    m->graph = g;
+    HIP_TRY(hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0));
+  }
+  *out = m;
+  return NLM_OK;
+}
+
+extern "C" int nlm_multi_launch(nlm_multi* m) {
+  if (!m) return set_err(NLM_INVALID_ARGUMENT, "null batch");
+  if (!m->exec) return NLM_OK;
+  HIP_TRY(hipEventRecord(m->t0, m->stream));
+  HIP_TRY(hipGraphLaunch(m->exec, m->stream));
+  HIP_TRY(hipEventRecord(m->t1, m->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_multi_sync(nlm_multi* m, double* ms) {
+  if (!m) return set_err(NLM_INVALID_ARGUMENT, "null batch");
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  if (ms) {
+    float f = 0.f;
+    if (m->exec) HIP_TRY(hipEventElapsedTime(&f, m->t0, m->t1));
+    *ms = f;
+  }
+  return NLM_OK;
+}
+
+extern "C" int nlm_multi_output(nlm_multi* m, int i, double* out) {
+  if (!m || !out) return set_err(NLM_INVALID_ARGUMENT, "null argument");
+  if (i < 0 || i >= m->n) return set_err(NLM_INVALID_ARGUMENT, "instance index out of range");
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  HIP_TRY(hipMemcpy(out, m->outBuf.p + m->outOff[i], sizeof(cplx) * m->nOut[i] * m->N[i], hipMemcpyDeviceToHost));
+  return NLM_OK;
+}
+
+extern "C" int nlm_multi_grid(nlm_multi* m, int c, int32_t* model, int32_t* log2n, int32_t* count, int32_t* grid) {
+  if (!m) return set_err(NLM_INVALID_ARGUMENT, "null batch");
+  if (c < 0 || c >= (int)m->classes.size()) return set_err(NLM_INVALID_ARGUMENT, "class index out of range");
+  if (model) *model = m->classes[c].model;
+  if (log2n) *log2n = m->classes[c].L;
+  if (count) *count = m->classes[c].count;
+  if (grid) *grid = m->classes[c].grid;
+  return NLM_OK;
+}
+
+extern "C" int nlm_multi_classes(nlm_multi* m) { return m ? (int)m->classes.size() : 0; }

@@ -50,6 +50,13 @@ struct KParams {
   cplx* ckpt;          // optional: time-domain field every ckptEvery steps, [nFields][nSeg][NS][N]
   int ckptEvery, nSeg;
   const cplx* tab;     // k_prop_tab: RK4 step propagators [nZ][N][2] (alpha, beta) / N
+  // pump rows generated in the kernel (linear pump, one pump): envelope [N]
+  // (time domain), pump dispersion D_p [N], dzp and beam-profile factors
+  // [2nZ-1] (or nullptr)
+  const cplx* env;
+  const double* Dp;
+  const double* strength;
+  double dzp;
 };
 
 template <int L>
@@ -175,17 +182,61 @@ __device__ __forceinline__ void epilogue(cplx (&a)[NS][8], const KParams& kp, cp
 // the split-step propagation kernel (pointwise models)
 // ---------------------------------------------------------------------------
 
-template <class M, int L>
-__global__ void __launch_bounds__(Launch<L>::THREADS) k_prop(KParams kp) {
+// Linear pump generated on the fly (_NonlinearMedium::runPumpSimulation,
+// src/_NonlinearMedium.cpp:225-250): row r = s_r IFFT(FFT(env) exp(i (r dzp) D_p)),
+// row 0 = s_0 env.  The phase advances row by row (one complex product) and is
+// re-evaluated exactly every 64 rows, which keeps it within ~1e-14 of the
+// direct evaluation.  Rows are produced in order.
+template <int L>
+struct InlinePump {
+  cplx F0[8], ph[8], e1[8];
+  int row;
+  NLM_DEV void init(const KParams& kp, cplx* buf, int t, const FftTwiddles<L>& tw, cplx (&row0)[8]) {
+    constexpr int T = FftShape<L>::T;
+    const double s0 = kp.strength ? kp.strength[0] : 1.0;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      F0[m] = kp.env[t + T * m];
+      row0[m] = s0 * F0[m];
+      double sn, cs;
+      sincos_reduced(kp.dzp * kp.Dp[t + T * m], &sn, &cs);
+      e1[m] = mk(cs, sn);
+      ph[m] = mk(1.0, 0.0);
+    }
+    fft_field<L, false>(F0, buf, t, tw);
+    row = 0;
+  }
+  NLM_DEV void next(const KParams& kp, cplx* buf, int t, const FftTwiddles<L>& tw, cplx (&out)[8]) {
+    constexpr int T = FftShape<L>::T;
+    row++;
+    const bool exact = (row & 63) == 0;
+    const double zr = (double)row * kp.dzp;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      if (exact) {
+        double sn, cs;
+        sincos_reduced(zr * kp.Dp[t + T * m], &sn, &cs);
+        ph[m] = mk(cs, sn);
+      } else {
+        ph[m] = ph[m] * e1[m];
+      }
+      out[m] = F0[m] * ph[m];
+    }
+    fft_field<L, true>(out, buf, t, tw);
+    const double s = (kp.strength ? kp.strength[row] : 1.0) * (1.0 / FftShape<L>::N);
+#pragma unroll
+    for (int m = 0; m < 8; m++) out[m] = out[m] * s;
+  }
+};
+
+// One propagation (the loop of signalSimulationTemplate) of field `field` on
+// the T lanes that hold it; INLINE generates the pump rows instead of reading
+// the pump grid.
+template <class M, int L, bool INLINE = false>
+__device__ __forceinline__ void prop_field(const KParams& kp, int field, bool active, int t, cplx* buf) {
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T, NS = M::NS, NP = M::NP, NC = M::NC;
-  constexpr int FPB = Launch<L>::FPB;
-  __shared__ cplx lds[FPB * F::lds_elems];
-
-  const int lf = threadIdx.x / T, t = threadIdx.x % T;
-  const int field = blockIdx.x * FPB + lf;
-  const bool active = field < kp.nFields;
-  cplx* buf = lds + lf * F::lds_elems;
+  static_assert(!INLINE || NP == 1, "the generated pump is a single linear pump");
 
   FftTwiddles<L> tw;
   fft_load_twiddles<L>(tw, kp.tw, t);
@@ -214,12 +265,21 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop(KParams kp) {
   // loaded before this step's transforms so their latency hides behind them
   cplx pp[NPP][8], pi[NPP][8], pc[NPP][8];
   const cplx* pumps[2] = {kp.pump0, kp.pump1};
-#pragma unroll
-  for (int r = 0; r < NP; r++) {
-    load_pump<L>(pp[r], pumps[r], t);
+  InlinePump<L> gen;
+  if constexpr (INLINE) {
+    gen.init(kp, buf, t, tw, pp[0]);
     if (kp.nZ > 1) {
-      load_pump<L>(pi[r], pumps[r] + (size_t)N, t);
-      load_pump<L>(pc[r], pumps[r] + (size_t)2 * N, t);
+      gen.next(kp, buf, t, tw, pi[0]);
+      gen.next(kp, buf, t, tw, pc[0]);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < NP; r++) {
+      load_pump<L>(pp[r], pumps[r], t);
+      if (kp.nZ > 1) {
+        load_pump<L>(pi[r], pumps[r] + (size_t)N, t);
+        load_pump<L>(pc[r], pumps[r] + (size_t)2 * N, t);
+      }
     }
   }
 
@@ -255,8 +315,13 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop(KParams kp) {
 #pragma unroll
       for (int m = 0; m < 8; m++) pp[r][m] = pc[r][m];
       if (i + 1 < kp.nZ) {
-        load_pump<L>(pi[r], pumps[r] + (size_t)(2 * i + 1) * N, t);
-        load_pump<L>(pc[r], pumps[r] + (size_t)(2 * i + 2) * N, t);
+        if constexpr (INLINE) {
+          gen.next(kp, buf, t, tw, pi[r]);
+          gen.next(kp, buf, t, tw, pc[r]);
+        } else {
+          load_pump<L>(pi[r], pumps[r] + (size_t)(2 * i + 1) * N, t);
+          load_pump<L>(pc[r], pumps[r] + (size_t)(2 * i + 2) * N, t);
+        }
       }
     }
 
@@ -277,6 +342,16 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop(KParams kp) {
     }
   }
   epilogue<L, NS>(a, kp, buf, t, tw, field, active);
+}
+
+template <class M, int L>
+__global__ void __launch_bounds__(Launch<L>::THREADS) k_prop(KParams kp) {
+  using F = FftShape<L>;
+  constexpr int T = F::T, FPB = Launch<L>::FPB;
+  __shared__ cplx lds[FPB * F::lds_elems];
+  const int lf = threadIdx.x / T, t = threadIdx.x % T;
+  const int field = blockIdx.x * FPB + lf;
+  prop_field<M, L>(kp, field, field < kp.nFields, t, lds + lf * F::lds_elems);
 }
 
 // ---------------------------------------------------------------------------

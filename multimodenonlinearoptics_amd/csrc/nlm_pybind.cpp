@@ -318,6 +318,41 @@ PYBIND11_MODULE(nonlinearmedium, m) {
            "inTimeDomain"_a = false, "runPump"_a = true, "nThreads"_a = 1, "normalize"_a = false,
            "useInput"_a = py::list(), "useOutput"_a = py::list());
 
+  // heterogeneous batch (extension)
+  py::class_<nlm::MultiBatch>(m, "MultiBatch",
+                              "One input row per medium, for many media of different models, sizes and parameters, "
+                              "propagated together by one graph launch (extension).  output(i) equals "
+                              "media[i].batchSignalSimulation(inputs[i][None], inTimeDomain)[0].")
+      .def(py::init([](const py::list& media, const py::list& inputs, bool inTimeDomain, bool autotune) {
+             std::vector<Medium*> v;
+             for (auto h : media) v.push_back(&h.cast<Medium&>());
+             std::vector<std::vector<cd>> in;
+             for (auto h : inputs) in.push_back(to_vec(h.cast<carr>()));
+             return new nlm::MultiBatch(v, in, inTimeDomain, autotune);
+           }),
+           "media"_a, "inputs"_a, "inTimeDomain"_a = false, "autotune"_a = true, py::keep_alive<1, 2>())
+      .def("launch", [](nlm::MultiBatch& b) {
+        py::gil_scoped_release nogil;
+        b.launch();
+      })
+      .def("sync", [](nlm::MultiBatch& b) {
+        py::gil_scoped_release nogil;
+        return b.sync();
+      }, "Wait for the last launch; returns its device time in ms.")
+      .def("run", [](nlm::MultiBatch& b) {
+        py::gil_scoped_release nogil;
+        return b.run();
+      }, "launch() then sync(); returns the device time in ms.")
+      .def("output", [](const nlm::MultiBatch& b, int i) {
+        std::vector<cd> o = b.output(i);
+        py::array_t<std::complex<double>> r(o.size());
+        std::copy(o.begin(), o.end(), r.mutable_data());
+        return r;
+      }, "i"_a)
+      .def("__len__", &nlm::MultiBatch::size)
+      .def("classes", &nlm::MultiBatch::classes,
+           "(model id, log2 nFreqs, instances, persistent workgroups) of each kernel class");
+
   // Cascade, src/nlmModulePy.cpp:170-211
   cascade.def(py::init([](const py::list& media, const std::vector<std::map<unsigned, unsigned>>& conns, bool share) {
                 std::vector<Medium*> v;

@@ -172,3 +172,38 @@ def config4(mod, N=1024, nZ=2000):
     m = mod.Chi2PDC(relativeLength=1.0, nlLength=1.0, beta2=b2, beta2s=b2s, beta1=0.0, beta1s=b1s, diffBeta0=0.0,
                     tMax=tMax, tPrecision=N, nZSteps=nZ, customPump=env, polingProfile=list(pol))
     return m, spec0, phi, pol
+
+
+def config3_instances(count=2000, seed=SEED0 + 3, sizes=(256, 512, 1024, 2048, 4096), nz_range=(200, 4000)):
+    """Heterogeneous batch: `count` Chi2SHG / Chi2PDC media (p = 1/2) with nFreqs drawn
+    from `sizes`, nZSteps ~ U{nz_range}, tMax=20, L=1, L_NL=1, beta1 ~ U(-2, 2),
+    beta2 ~ U(-1, 1) per mode; the pulse (SHG: the fundamental input; PDC: the pump)
+    is a Gaussian or sech (p = 1/2) of width U(0.3, 3) with spectral phase
+    a omega + b omega^2, a, b ~ U(-1, 1); the PDC signal input is exp(-tau^2/2).
+
+    Returns a list of (class name, kwargs, time-domain input row)."""
+    r = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        name = "Chi2SHG" if r.uniform() < 0.5 else "Chi2PDC"
+        N = int(r.choice(sizes))
+        nZ = int(r.integers(nz_range[0], nz_range[1] + 1))
+        tMax = 20.0
+        tau = 2 * tMax / N * np.fft.fftshift(np.arange(-N // 2, N // 2))
+        omega = -np.pi / tMax * np.fft.fftshift(np.arange(-N // 2, N // 2))
+        w = r.uniform(0.3, 3.0)
+        shape = np.exp(-0.5 * (tau / w) ** 2) if r.uniform() < 0.5 else 1 / np.cosh(tau / w)
+        a, b = r.uniform(-1, 1, size=2)
+        pulse = np.fft.ifft(np.fft.fft(shape.astype(np.complex128)) * np.exp(1j * (a * omega + b * omega ** 2)))
+        b1 = r.uniform(-2, 2, size=2)
+        b2 = r.uniform(-1, 1, size=2)
+        if name == "Chi2SHG":
+            kw = dict(relativeLength=1.0, nlLengthH=1.0, nlLengthP=1.0, beta2h=float(b2[0]), beta2p=float(b2[1]),
+                      beta1h=float(b1[0]), beta1p=float(b1[1]), tMax=tMax, tPrecision=N, nZSteps=nZ)
+            x = pulse
+        else:
+            kw = dict(relativeLength=1.0, nlLength=1.0, beta2=float(b2[0]), beta2s=float(b2[1]), beta1=float(b1[0]),
+                      beta1s=float(b1[1]), tMax=tMax, tPrecision=N, nZSteps=nZ, customPump=pulse)
+            x = np.exp(-tau ** 2 / 2).astype(np.complex128)
+        out.append((name, kw, x))
+    return out

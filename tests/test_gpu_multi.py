@@ -1,0 +1,64 @@
+"""Heterogeneous batches (MultiBatch, nlm_multi_*): every instance against the
+oracle's batchSignalSimulation of the same medium and input (rel. L2 <= 1e-10),
+and against the engine's own per-medium path."""
+import numpy as np
+import pytest
+
+from cases import config3_instances
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _build(mod, insts):
+    return [getattr(mod, name)(**kw) for name, kw, _ in insts]
+
+
+def test_mixed_batch_matches_oracle(nlm, oracle):
+    insts = config3_instances(count=14, seed=77, sizes=(32, 64, 128, 256, 512), nz_range=(20, 90))
+    media = _build(nlm, insts)
+    ref = _build(oracle, insts)
+    xs = [x for _, _, x in insts]
+    b = nlm.MultiBatch(media, xs, inTimeDomain=True)
+    assert len(b) == len(insts)
+    ms = b.run()
+    assert ms > 0
+    for i, (m, r, x) in enumerate(zip(media, ref, xs)):
+        got = b.output(i)
+        exp = r.batchSignalSimulation(x[None, :], inTimeDomain=True)[0]
+        assert got.shape == exp.shape
+        assert rel_l2(got, exp) < TOL, (i, insts[i][0], insts[i][1]["tPrecision"], rel_l2(got, exp))
+        own = m.batchSignalSimulation(x[None, :], inTimeDomain=True)[0]
+        assert rel_l2(got, own) < 1e-12
+    # a second launch of the captured graph gives the same outputs
+    first = [b.output(i) for i in range(len(b))]
+    b.run()
+    for i in range(len(b)):
+        assert np.array_equal(first[i], b.output(i))
+    cls = b.classes()
+    assert sum(c[2] for c in cls) == len(insts)
+    assert all(1 <= c[3] <= c[2] for c in cls)
+
+
+def test_mixed_batch_frequency_domain_and_sizes(nlm, oracle):
+    insts = config3_instances(count=6, seed=5, sizes=(1024, 2048, 4096), nz_range=(6, 12))
+    media = _build(nlm, insts)
+    ref = _build(oracle, insts)
+    xs = [np.fft.fft(x) for _, _, x in insts]
+    b = nlm.MultiBatch(media, xs, inTimeDomain=False, autotune=False)
+    b.run()
+    for i, (r, x) in enumerate(zip(ref, xs)):
+        exp = r.batchSignalSimulation(x[None, :], inTimeDomain=False)[0]
+        assert rel_l2(b.output(i), exp) < TOL, i
+
+
+def test_mixed_batch_errors(nlm):
+    chi3 = nlm.Chi3(relativeLength=1, nlLength=1, beta2=-1, tMax=10, tPrecision=64, nZSteps=10)
+    with pytest.raises(ValueError, match="heterogeneous batches support"):
+        nlm.MultiBatch([chi3], [np.zeros(64, complex)])
+    pdc = nlm.Chi2PDC(relativeLength=1, nlLength=1, beta2=0.1, beta2s=0.1, tMax=10, tPrecision=64, nZSteps=10)
+    with pytest.raises(ValueError, match="Signals not of correct length"):
+        nlm.MultiBatch([pdc], [np.zeros(60, complex)])
+    with pytest.raises(ValueError):
+        nlm.MultiBatch([pdc, pdc], [np.zeros(64, complex)])
