@@ -131,6 +131,74 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6):
     }
 
 
+def flops_shg(n):
+    """Algorithmic FP64 flops of one Chi2SHG field-step (two modes)."""
+    return 2 * (10 * n * math.log2(n) + 8 * n) + 154 * n
+
+
+def hetero_benchmark(nlm, count=2000, reps=3, plain_sample=40, cpu_sample=3):
+    """Heterogeneous batch: `count` Chi2SHG / Chi2PDC media of mixed nFreqs (256..4096)
+    and nZSteps (200..4000), one input row each, run as one MultiBatch graph launch.
+
+    Reports field-steps/s of the whole batch (inputs resident in HBM, device time of
+    the graph launch), the same instances run one by one through
+    batchSignalSimulation (sampled, scaled by algorithmic cost), the C oracle on a
+    few instances, and the worst rel. L2 error of sampled outputs against it."""
+    import numpy as np
+    from cases import config3_instances
+    from oracle import oracle as O
+    insts = config3_instances(count)
+    t0 = time.perf_counter()
+    media = [getattr(nlm, name)(**kw) for name, kw, _ in insts]
+    xs = [x for _, _, x in insts]
+    b = nlm.MultiBatch(media, xs, inTimeDomain=True, autotune=True)
+    t_create = time.perf_counter() - t0
+    b.run()
+    dev_ms, wall = [], []
+    for _ in range(reps):
+        w0 = time.perf_counter()
+        dev_ms.append(b.run())
+        wall.append(time.perf_counter() - w0)
+    steps = [kw["nZSteps"] - 1 for _, kw, _ in insts]
+    cost = [(flops_shg(kw["tPrecision"]) if name == "Chi2SHG" else flops_per_field_step(kw["tPrecision"])) * s
+            for (name, kw, _), s in zip(insts, steps)]
+    total_steps, total_flops = sum(steps), sum(cost)
+    ms = min(dev_ms)
+    # plain: one instance per call, a uniform sample scaled by algorithmic cost
+    r = np.random.default_rng(1)
+    idx = sorted(r.choice(count, size=min(plain_sample, count), replace=False).tolist())
+    media[idx[0]].batchSignalSimulation(xs[idx[0]][None, :], inTimeDomain=True)
+    p0 = time.perf_counter()
+    for i in idx:
+        media[i].batchSignalSimulation(xs[i][None, :], inTimeDomain=True)
+    plain_s = (time.perf_counter() - p0) * total_flops / sum(cost[i] for i in idx)
+    # oracle on the cheapest instances: parity and CPU throughput
+    cheap = sorted(range(count), key=lambda i: cost[i])[:cpu_sample]
+    worst, cpu_t, cpu_steps = 0.0, 0.0, 0
+    for i in cheap:
+        name, kw, x = insts[i]
+        ref = getattr(O, name)(**kw)
+        c0 = time.perf_counter()
+        exp = ref.batchSignalSimulation(x[None, :], inTimeDomain=True)[0]
+        cpu_t += time.perf_counter() - c0
+        cpu_steps += steps[i]
+        got = b.output(i)
+        worst = max(worst, float(np.linalg.norm(got - exp) / np.linalg.norm(exp)))
+    classes = [list(c) for c in b.classes()]
+    return {
+        "workload": "%d Chi2SHG/Chi2PDC instances, nFreqs in {256..4096}, nZSteps U[200,4000], one graph launch"
+                    % count,
+        "instances": count, "field_steps": total_steps, "algorithmic_flops": total_flops,
+        "ms": ms, "wall_ms": min(wall) * 1e3, "create_s": t_create,
+        "value": total_steps / (ms * 1e-3), "unit": "field-steps/s",
+        "tflops": total_flops / (ms * 1e-3) / 1e12,
+        "plain_s": plain_s, "plain_sample": len(idx), "batched_vs_plain": plain_s / (ms * 1e-3),
+        "cpu": {"field_steps_per_s": cpu_steps / cpu_t, "cores": 1, "kind": "port",
+                "sample": "%d cheapest instances on one thread" % len(cheap)},
+        "rel_err_max": worst, "classes": classes,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +207,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint (gradient) measurement")
+    ap.add_argument("--no-hetero", action="store_true", help="skip the heterogeneous-batch measurement")
+    ap.add_argument("--hetero-count", type=int, default=2000)
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (default: min(16, cores))")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="only run a device-to-device copy of this many MiB (calibrates HBM byte counters)")
@@ -231,6 +301,9 @@ def main():
                      "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
                      "kernel": "k_prop<MChi2PDC, 9>", "flops_per_field_step": fl},
     }
+
+    if not args.no_hetero:
+        rec["hetero"] = hetero_benchmark(nlm, count=args.hetero_count)
 
     if not args.no_grad:
         g = grad_benchmark(nlm)

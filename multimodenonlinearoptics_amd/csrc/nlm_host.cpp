@@ -167,17 +167,23 @@ void Medium::resetGrids(unsigned nFreqs, double tMax) {
   }
   tau_ = fftshift(tau_);
   omega_ = fftshift(omega_);
-  pumpFreq_.resize(nP_);
-  pumpTime_.resize(nP_);
-  for (int m = 0; m < nP_; m++) {
-    pumpFreq_[m].setZero(nZP_, N_);
-    pumpTime_[m].setZero(nZP_, N_);
-  }
-  signalFreq_.resize(nS_);
-  signalTime_.resize(nS_);
+  // host copies of the grids are allocated on first use (the device holds the
+  // working grids; a batch of many media never touches these)
+  pumpFreq_.assign(nP_, Grid());
+  pumpTime_.assign(nP_, Grid());
+  signalFreq_.assign(nS_, Grid());
+  signalTime_.assign(nS_, Grid());
+}
+
+void Medium::allocPump(unsigned q) {
+  if (pumpFreq_[q].rows != nZP_ || pumpFreq_[q].cols != N_) pumpFreq_[q].setZero(nZP_, N_);
+  if (pumpTime_[q].rows != nZP_ || pumpTime_[q].cols != N_) pumpTime_[q].setZero(nZP_, N_);
+}
+
+void Medium::allocSignal() {
   for (int m = 0; m < nS_; m++) {
-    signalFreq_[m].setZero(nZ_, N_);
-    signalTime_[m].setZero(nZ_, N_);
+    if (signalFreq_[m].rows != nZ_ || signalFreq_[m].cols != N_) signalFreq_[m].setZero(nZ_, N_);
+    if (signalTime_[m].rows != nZ_ || signalTime_[m].cols != N_) signalTime_[m].setZero(nZ_, N_);
   }
 }
 
@@ -393,6 +399,7 @@ void Medium::refreshCoef() {
 }
 
 void Medium::uploadPump(unsigned q) {
+  allocPump(q);
   check(nlm_pump_upload(plan(), (int)q, reinterpret_cast<const double*>(pumpTime_[q].v.data()),
                         reinterpret_cast<const double*>(pumpFreq_[q].v.data())));
   pumpDevStale_[q] = false;
@@ -407,6 +414,7 @@ void Medium::markPumpOnDevice() {
 }
 
 void Medium::syncPumpToHost(unsigned i) {
+  if ((int)i < nP_) allocPump(i);
   if (i < pumpHostStale_.size() && pumpHostStale_[i] && plan_) {
     check(nlm_pump_download(plan_, (int)i, reinterpret_cast<double*>(pumpTime_[i].v.data()),
                             reinterpret_cast<double*>(pumpFreq_[i].v.data())));
@@ -415,6 +423,7 @@ void Medium::syncPumpToHost(unsigned i) {
 }
 
 void Medium::syncSignalToHost() {
+  allocSignal();
   if (signalHostStale_ && plan_) {
     for (int m = 0; m < nS_; m++)
       check(nlm_grid_download(plan_, m, reinterpret_cast<double*>(signalFreq_[m].v.data()),

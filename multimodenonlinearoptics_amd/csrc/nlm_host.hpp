@@ -169,6 +169,8 @@ class Medium {
   std::vector<std::vector<double>> dispPump_, dispSign_;
   std::vector<std::vector<cd>> dispStepPump_, dispStepSign_;
   std::vector<Grid> pumpFreq_, pumpTime_, signalFreq_, signalTime_, field_;
+  void allocPump(unsigned q);
+  void allocSignal();
   std::vector<cd> raman_;
 
   nlm_plan* plan_ = nullptr;
