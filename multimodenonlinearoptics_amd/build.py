@@ -65,6 +65,11 @@ def build(force=False, jobs=None, verbose=False):
     deps = _deps("nlm_adjoint.hip", "nlm_adjoint.hpp", "nlm_internal.hpp", *dev_hdrs)
     if force or _stale(o, deps):
         steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_adjoint.hip"), "-o", o])
+    o = os.path.join(OBJ, "nlm_big.o")
+    objs.append(o)
+    deps = _deps("nlm_big.hip", "nlm_internal.hpp", *dev_hdrs)
+    if force or _stale(o, deps):
+        steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_big.hip"), "-o", o])
     o = os.path.join(OBJ, "nlm_multi.o")
     objs.append(o)
     deps = _deps("nlm_multi.hip", "nlm_internal.hpp", *dev_hdrs)

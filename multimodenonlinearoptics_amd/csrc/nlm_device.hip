@@ -377,7 +377,11 @@ extern "C" int nlm_plan_create(const nlm_plan_desc* d, nlm_plan** out) {
   if (!d) return set_err(NLM_INVALID_ARGUMENT, "null plan descriptor");
   if (d->model < 0 || d->model >= NLM_NUM_MODELS) return set_err(NLM_INVALID_ARGUMENT, "unknown model");
   int L;
-  if (int e = check_n(d->n_freqs, &L)) return e;
+  if (d->model == NLM_CHI3GNLSE && (d->n_freqs == 16384 || d->n_freqs == 32768)) {
+    L = log2_exact(d->n_freqs);  // staged transform (batches only)
+  } else if (int e = check_n(d->n_freqs, &L)) {
+    return e;
+  }
   if (d->n_zsteps < 1) return set_err(NLM_INVALID_ARGUMENT, "Zero steps");
   if (d->model == NLM_CHI3GNLSE && (!d->raman || !d->gnlse_filter))
     return set_err(NLM_INVALID_ARGUMENT, "Chi3GNLSE plan needs the Raman spectrum and derivative filter");
@@ -430,6 +434,10 @@ extern "C" int nlm_plan_destroy(nlm_plan* p) {
   DevBuf<cplx>* abufs[] = {&p->ckpt, &p->gG, &p->gC, &p->gS, &p->gcoefPart, &p->gcoef, &p->gpumpPart, &p->gpump,
                            &p->gF0Part, &p->gF0, &p->gtmp, &p->scale, &p->stepTab, &p->stepInv};
   for (auto* b : abufs) b->release();
+  if (p->bigExec) (void)hipGraphExecDestroy(p->bigExec);
+  p->bigState.release();
+  p->bigH.release();
+  p->bigBase.release();
   p->Dp.release();
   p->strength.release();
   for (auto& ev : p->ev)
@@ -596,6 +604,8 @@ void input_channels(int nS, int nCh, int inputMode, int* inPos) {
 }
 
 int launch_prop_kernel(nlm_plan* p, const KParams& kp0) {
+  if (p->L > 13)
+    return set_err(NLM_INVALID_ARGUMENT, "nFreqs above 8192 is available for Chi3GNLSE batchSignalSimulation only");
   KParams kp = kp0;
   prop_fn fn = nullptr;
   if (p->stepTable && (p->model == NLM_CHI2PDC || p->model == NLM_CHI3)) {
@@ -686,6 +696,15 @@ static int batch_common(nlm_plan* p, const cplx* in_dev, int rows, int cols, int
   }
   input_channels(p->nS, cols / N, input_mode, kp.inPos);
   if (rows == 0) return NLM_OK;
+  if (nlm_big_supported(p->model, p->L)) {
+    tic(p, 2);
+    if (int e = nlm_big_batch(p, in_dev, rows, cols, in_time, kp.inPos[0] < 0 ? 0 : kp.inPos[0], n_out,
+                              kp.outPos[0] < 0 ? 0 : kp.outPos[0], out_dev))
+      return e;
+    tic(p, 3);
+    if (p->timing) collect_timing(p, 2, 3, 1);
+    return NLM_OK;
+  }
   tic(p, 2);
   if (int e = launch_prop_kernel(p, kp)) return e;
   tic(p, 3);

@@ -66,6 +66,14 @@ struct nlm_plan_s {
   DevBuf<cplx> ckpt, gG, gC, gS, gcoefPart, gcoef, gpumpPart, gpump, gF0Part, gF0, gtmp;
   DevBuf<cplx> scale;
   DevBuf<cplx> stepInv;  // inverse RK4 step propagators (adjoint)
+  // staged transform for Chi3GNLSE with nFreqs > 8192 (nlm_big.hip)
+  DevBuf<cplx> bigState, bigH;
+  DevBuf<int> bigBase;
+  hipGraphExec_t bigExec = nullptr;
+  int bigKey[6] = {0, 0, 0, 0, 0, 0};
+  const void* bigIn = nullptr;
+  const void* bigOut = nullptr;
+  const void* bigStatePtr = nullptr;
   DevBuf<cplx> stepTab;  // RK4 step propagators of one-mode undepleted media [nZ][N][2]
   int stepTable = 1;      // use them (nlm_plan_set_step_table)
   int ckptEvery = 0, nSeg = 0, gNIn = 0, gNOut = 0, gInTime = 0, gHasScale = 0;
@@ -77,6 +85,10 @@ int launch_prop_kernel(nlm_plan_s* p, const KParams& kp);
 int get_twiddles(int n, const cplx** out);
 // input channel feeding each mode (-1: none), src/_NonlinearMedium.hpp:164-198
 void input_channels(int nS, int nCh, int inputMode, int* inPos);
+// Chi3GNLSE batches with nFreqs 16384 / 32768 (nlm_big.hip)
+bool nlm_big_supported(int model, int L);
+int nlm_big_batch(nlm_plan_s* p, const cplx* in, int rows, int cols, int inTime, int inPos, int nOut, int outPos,
+                  cplx* out);
 void tic(nlm_plan_s* p, int slot);
 void collect_timing(nlm_plan_s* p, int a0, int a1, int slot);
 void launch_greens_assemble(const cplx* G, cplx* C, cplx* S, int N, int nIn, int nOut, int nb, const cplx* scale,
