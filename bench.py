@@ -199,6 +199,47 @@ def hetero_benchmark(nlm, count=2000, reps=3, plain_sample=40, cpu_sample=3):
     }
 
 
+def flops_gnlse(n):
+    """Algorithmic FP64 flops of one Chi3GNLSE field-step (16 RK transforms + 2)."""
+    return 90 * n * math.log2(n) + 154 * n
+
+
+def gnlse_benchmark(nlm, rows=64, nz=20000, n=16384, cpu_steps=6):
+    """Chi3GNLSE, nFreqs=16384, tMax=50, nZSteps=20000, batch of 64 sech pulses
+    (soliton order 3) with seeded one-photon noise spectra: one batchSignalSimulation,
+    transforms staged through HBM.  The oracle times a few steps of one field."""
+    import numpy as np
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from test_gpu_big import config5_kwargs, inputs
+    kw = config5_kwargs(n, nz)
+    X = inputs(n, rows, seed=20261202 + 5)
+    m = nlm.Chi3GNLSE(**kw)
+    m.enableTiming(True)
+    t0 = time.perf_counter()
+    m.batchSignalSimulation(X, inTimeDomain=True)
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Y = m.batchSignalSimulation(X, inTimeDomain=True)
+    wall = time.perf_counter() - t0
+    dev_ms = m.lastTiming()[1]
+    steps = rows * (nz - 1)
+    kwc = dict(kw, nZSteps=cpu_steps + 1, relativeLength=kw["relativeLength"] * cpu_steps / (nz - 1))
+    ref = O.Chi3GNLSE(**kwc)
+    c0 = time.perf_counter()
+    ref.batchSignalSimulation(X[:1], inTimeDomain=True)
+    cpu_s = time.perf_counter() - c0
+    return {
+        "workload": "Chi3GNLSE nFreqs=16384 tMax=50 nZSteps=20000, batch of 64 sech (N=3) + noise",
+        "field_steps": steps, "device_ms": dev_ms, "wall_s": wall, "first_call_s": t_first,
+        "value": steps / (dev_ms * 1e-3), "unit": "field-steps/s",
+        "tflops": flops_gnlse(n) * steps / (dev_ms * 1e-3) / 1e12,
+        "finite": bool(np.isfinite(Y).all()),
+        "cpu": {"field_steps_per_s": cpu_steps / cpu_s, "cores": 1, "kind": "port",
+                "sample": "%d steps of one field" % cpu_steps},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +250,7 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint (gradient) measurement")
     ap.add_argument("--no-hetero", action="store_true", help="skip the heterogeneous-batch measurement")
     ap.add_argument("--hetero-count", type=int, default=2000)
+    ap.add_argument("--no-gnlse", action="store_true", help="skip the nFreqs=16384 Chi3GNLSE measurement")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (default: min(16, cores))")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="only run a device-to-device copy of this many MiB (calibrates HBM byte counters)")
@@ -299,11 +341,14 @@ def main():
         "grad": None,
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                     "kernel": "k_prop<MChi2PDC, 9>", "flops_per_field_step": fl},
+                     "kernel": "k_prop_tab<9> (Chi2PDC Green's launch)", "flops_per_field_step": fl},
     }
 
     if not args.no_hetero:
         rec["hetero"] = hetero_benchmark(nlm, count=args.hetero_count)
+
+    if not args.no_gnlse:
+        rec["gnlse"] = gnlse_benchmark(nlm)
 
     if not args.no_grad:
         g = grad_benchmark(nlm)
