@@ -207,12 +207,15 @@ def flops_gnlse(n):
 def gnlse_benchmark(nlm, rows=64, nz=20000, n=16384, cpu_steps=6):
     """Chi3GNLSE, nFreqs=16384, tMax=50, nZSteps=20000, batch of 64 sech pulses
     (soliton order 3) with seeded one-photon noise spectra: one batchSignalSimulation,
-    transforms staged through HBM.  The oracle times a few steps of one field."""
+    transforms staged through HBM.  The oracle times a few steps of one field.
+    A smaller nz shortens the medium in proportion (same step size dz), which
+    keeps the run stable for profiling."""
     import numpy as np
     from oracle import oracle as O
     sys.path.insert(0, os.path.join(HERE, "tests"))
     from test_gpu_big import config5_kwargs, inputs
     kw = config5_kwargs(n, nz)
+    kw["relativeLength"] *= (nz - 1) / 19999.0
     X = inputs(n, rows, seed=20261202 + 5)
     m = nlm.Chi3GNLSE(**kw)
     m.enableTiming(True)
@@ -230,13 +233,63 @@ def gnlse_benchmark(nlm, rows=64, nz=20000, n=16384, cpu_steps=6):
     ref.batchSignalSimulation(X[:1], inTimeDomain=True)
     cpu_s = time.perf_counter() - c0
     return {
-        "workload": "Chi3GNLSE nFreqs=16384 tMax=50 nZSteps=20000, batch of 64 sech (N=3) + noise",
+        "workload": "Chi3GNLSE nFreqs=%d tMax=50 nZSteps=%d, batch of %d sech (N=3) + noise" % (n, nz, rows),
         "field_steps": steps, "device_ms": dev_ms, "wall_s": wall, "first_call_s": t_first,
         "value": steps / (dev_ms * 1e-3), "unit": "field-steps/s",
         "tflops": flops_gnlse(n) * steps / (dev_ms * 1e-3) / 1e12,
         "finite": bool(np.isfinite(Y).all()),
         "cpu": {"field_steps_per_s": cpu_steps / cpu_s, "cores": 1, "kind": "port",
                 "sample": "%d steps of one field" % cpu_steps},
+    }
+
+
+def gnlse_grad_benchmark(nlm, rows=64, nz=20000, n=16384, fwd_ms=None, fd_dirs=2, fd_eps=1e-6):
+    """Gradient of the spectral energy in omega in [5, 15] of the Chi3GNLSE batch
+    above w.r.t. its 64 input fields: checkpointed forward (batchForward), then
+    the adjoint (batchAdjoint); central differences of the plain forward along
+    fd_dirs random unit directions check it."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from test_gpu_big import config5_kwargs, inputs
+    kw = config5_kwargs(n, nz)
+    kw["relativeLength"] *= (nz - 1) / 19999.0
+    X = inputs(n, rows, seed=20261202 + 5)
+    m = nlm.Chi3GNLSE(**kw)
+    m.enableTiming(True)
+    band = (np.asarray(m.omega) >= 5) & (np.asarray(m.omega) <= 15)
+
+    def loss_of(Y):
+        F = np.fft.fft(Y, axis=1)
+        return float(np.sum(np.abs(F[:, band]) ** 2)), F
+
+    t0 = time.perf_counter()
+    Y = m.batchForward(X, inTimeDomain=True)
+    t_fwd = time.perf_counter() - t0
+    fwd_ckpt_ms = m.lastTiming()[1]
+    L0, F = loss_of(Y)
+    gY = n * np.fft.ifft(2 * F * band[None, :], axis=1)
+    t0 = time.perf_counter()
+    g = m.batchAdjoint(gY)
+    t_bwd = time.perf_counter() - t0
+    bwd_ms = m.lastTiming()[1]
+    r = np.random.default_rng(20261202 + 6)
+    pairs = []
+    gn = float(np.linalg.norm(g))
+    for _ in range(fd_dirs):
+        d = r.standard_normal(X.shape) + 1j * r.standard_normal(X.shape)
+        d /= np.linalg.norm(d)
+        lp, _ = loss_of(m.batchSignalSimulation(X + fd_eps * d, inTimeDomain=True))
+        lm, _ = loss_of(m.batchSignalSimulation(X - fd_eps * d, inTimeDomain=True))
+        pairs.append([(lp - lm) / (2 * fd_eps), float(np.real(np.vdot(g, d)))])
+    base = fwd_ms if fwd_ms else fwd_ckpt_ms
+    return {
+        "workload": "d(spectral energy in omega in [5,15])/d(inputs), Chi3GNLSE nFreqs=%d nZSteps=%d, batch %d"
+                    % (n, nz, rows),
+        "loss": L0, "forward_ckpt_ms": fwd_ckpt_ms, "backward_ms": bwd_ms, "forward_ckpt_s": t_fwd,
+        "backward_s": t_bwd, "grad_vs_forward": bwd_ms / base, "grad_norm": gn,
+        "fd_eps": fd_eps, "fd_pairs": pairs,
+        "fd_max_err_over_grad_norm": max(abs(a - b) for a, b in pairs) / gn if gn > 0 else None,
+        "finite": bool(np.isfinite(g).all()),
     }
 
 
@@ -251,6 +304,11 @@ def main():
     ap.add_argument("--no-hetero", action="store_true", help="skip the heterogeneous-batch measurement")
     ap.add_argument("--hetero-count", type=int, default=2000)
     ap.add_argument("--no-gnlse", action="store_true", help="skip the nFreqs=16384 Chi3GNLSE measurement")
+    ap.add_argument("--no-gnlse-grad", action="store_true", help="skip the Chi3GNLSE input-gradient measurement")
+    ap.add_argument("--only", choices=["hetero", "gnlse", "grad", "gnlse_grad"], default=None,
+                    help="run only this section and print its record (for profiling)")
+    ap.add_argument("--gnlse-nz", type=int, default=20000,
+                    help="nZSteps of the Chi3GNLSE section (the medium is shortened in proportion)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (default: min(16, cores))")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="only run a device-to-device copy of this many MiB (calibrates HBM byte counters)")
@@ -275,6 +333,17 @@ def main():
     from cases import config2
 
     nlm = pkg.nonlinearmedium
+    if args.only:
+        if args.only == "hetero":
+            out = hetero_benchmark(nlm, count=args.hetero_count)
+        elif args.only == "gnlse":
+            out = gnlse_benchmark(nlm, nz=args.gnlse_nz)
+        elif args.only == "gnlse_grad":
+            out = gnlse_grad_benchmark(nlm, nz=args.gnlse_nz)
+        else:
+            out = grad_benchmark(nlm)
+        print(json.dumps(out))
+        return
     medium = config2(nlm, N_FREQS, NZ)
     medium.enableTiming(True)
     field_steps = 2 * N_FREQS * (NZ - 1)
@@ -348,7 +417,11 @@ def main():
         rec["hetero"] = hetero_benchmark(nlm, count=args.hetero_count)
 
     if not args.no_gnlse:
-        rec["gnlse"] = gnlse_benchmark(nlm)
+        rec["gnlse"] = gnlse_benchmark(nlm, nz=args.gnlse_nz)
+
+    if not args.no_gnlse_grad:
+        rec["gnlse_grad"] = gnlse_grad_benchmark(nlm, nz=args.gnlse_nz,
+                                                 fwd_ms=rec["gnlse"]["device_ms"] if "gnlse" in rec else None)
 
     if not args.no_grad:
         g = grad_benchmark(nlm)

@@ -200,6 +200,23 @@ int nlm_greens_forward(nlm_plan* plan, int in_time_domain, int n_in, const int32
 int nlm_greens_adjoint(nlm_plan* plan, const double* gC, const double* gS, double* g_coef, double* g_pump_time,
                        double* g_envelope);
 
+/* Differentiable Chi3GNLSE batch (batchSignalSimulation of a fully nonlinear
+ * medium, src/_FullyNonlinearMedium.hpp:8-11 with Chi3GNLSE::DiffEq,
+ * solvers/Chi3GNLSE.cpp:39-92, inside signalSimulationTemplate,
+ * src/_NonlinearMedium.hpp:160-239).  nlm_batch_forward is nlm_batch for rows
+ * input rows of length N (one mode, output in the input's domain) that also
+ * keeps the time-domain field every `ckpt_every` z steps (<= 0:
+ * ceil(sqrt((nZ-1)/8))) on the device.  nlm_batch_adjoint then takes dL/d(out)
+ * (rows x N complex, convention dL/dRe + i dL/dIm, the layout and domain of
+ * out) and returns dL/d(inputs) (rows x N, the domain of the inputs).  The
+ * reverse sweep recomputes each checkpoint segment once, keeping the stage
+ * inputs and Raman convolutions of its steps: device memory
+ * (nSeg + 8 ckpt_every + 10) x rows x N x 16 bytes.  nFreqs 32..32768.
+ * (Extension: the reference has no derivatives.) */
+int nlm_batch_forward(nlm_plan* plan, const double* inputs, int rows, int in_time_domain, int ckpt_every,
+                      double* out);
+int nlm_batch_adjoint(nlm_plan* plan, const double* g_out, double* g_in);
+
 /* Green's function composition of two media, Cascade::computeGreensFunction
  * (src/Cascade.cpp:128-129): C = C2 C1 + S2 conj(S1), S = C2 S1 + S2 conj(C1).
  * C2, S2: M x K; C1, S1: K x Ncol; C, S: M x Ncol (host, row-major). */

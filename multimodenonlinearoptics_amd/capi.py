@@ -17,7 +17,7 @@ SYMBOLS = [
     "nlm_batch_dev", "nlm_plan_set_step_table", "nlm_plan_enable_timing", "nlm_plan_last_timing", "nlm_greens_compose", "nlm_fft",
     "nlm_plan_sync", "nlm_last_error", "nlm_version", "nlm_greens_forward", "nlm_greens_adjoint",
     "nlm_multi_create", "nlm_multi_launch", "nlm_multi_sync", "nlm_multi_output", "nlm_multi_classes",
-    "nlm_multi_grid", "nlm_multi_destroy",
+    "nlm_multi_grid", "nlm_multi_destroy", "nlm_batch_forward", "nlm_batch_adjoint",
 ]
 
 _lib = None

@@ -579,6 +579,31 @@ std::pair<std::vector<cd>, std::vector<double>> Medium::greensAdjoint(const Grid
   return {genv, gpol};
 }
 
+// Differentiable batchSignalSimulation of a fully nonlinear medium
+// (src/_FullyNonlinearMedium.hpp:8-11) for Chi3GNLSE.
+Grid Medium::batchForward(const Grid& in, bool inTimeDomain, int ckptEvery) {
+  if (model_ != NLM_CHI3GNLSE) throw std::invalid_argument("the differentiable batch is implemented for Chi3GNLSE");
+  if (in.cols != N_) throw std::invalid_argument("Signals not of correct length!");
+  if (in.rows == 0) throw std::invalid_argument("empty batch");
+  Grid out;
+  out.setZero(in.rows, N_);
+  check(nlm_batch_forward(plan(), reinterpret_cast<const double*>(in.v.data()), in.rows, inTimeDomain ? 1 : 0,
+                          ckptEvery, reinterpret_cast<double*>(out.v.data())));
+  batchRows_ = in.rows;
+  return out;
+}
+
+Grid Medium::batchAdjoint(const Grid& gOut) {
+  if (model_ != NLM_CHI3GNLSE) throw std::invalid_argument("the differentiable batch is implemented for Chi3GNLSE");
+  if (batchRows_ == 0) throw std::runtime_error("run batchForward before batchAdjoint");
+  if (gOut.rows != batchRows_ || gOut.cols != N_)
+    throw std::invalid_argument("gradient shape does not match the last batchForward");
+  Grid g;
+  g.setZero(gOut.rows, N_);
+  check(nlm_batch_adjoint(plan(), reinterpret_cast<const double*>(gOut.v.data()), reinterpret_cast<double*>(g.v.data())));
+  return g;
+}
+
 void Medium::enableTiming(bool on) { check(nlm_plan_enable_timing(plan(), on ? 1 : 0)); }
 void Medium::setStepTable(bool on) { check(nlm_plan_set_step_table(plan(), on ? 1 : 0)); }
 

@@ -102,6 +102,9 @@ class Medium {
   // returns (dL/d envelope [N] complex, dL/d poling [nZ] real) for dL/dC, dL/dS
   std::pair<Grid, Grid> greensForward(bool inTimeDomain, bool runPump, int ckptEvery);
   std::pair<std::vector<cd>, std::vector<double>> greensAdjoint(const Grid& gC, const Grid& gS);
+  // differentiable batch (Chi3GNLSE): forward keeping checkpoints, then dL/d(inputs)
+  Grid batchForward(const Grid& inputProfs, bool inTimeDomain, int ckptEvery);
+  Grid batchAdjoint(const Grid& gOut);
   void enableTiming(bool on);
   void setStepTable(bool on);
   std::vector<double> lastTiming();
@@ -179,6 +182,7 @@ class Medium {
   std::vector<bool> pumpHostStale_, pumpDevStale_;
   bool signalHostStale_ = false;
   bool pumpComputed_ = false;
+  int batchRows_ = 0;  // rows of the last batchForward
 };
 
 class FullyNonlinearMedium : public Medium {

@@ -77,6 +77,9 @@ struct nlm_plan_s {
   DevBuf<cplx> stepTab;  // RK4 step propagators of one-mode undepleted media [nZ][N][2]
   int stepTable = 1;      // use them (nlm_plan_set_step_table)
   int ckptEvery = 0, nSeg = 0, gNIn = 0, gNOut = 0, gInTime = 0, gHasScale = 0;
+  // differentiable Chi3GNLSE batch (nlm_batch_forward / nlm_batch_adjoint)
+  int bRows = 0;
+  DevBuf<cplx> adjWork, adjSeg;
 };
 
 void fill_common(nlm_plan_s* p, KParams& kp);

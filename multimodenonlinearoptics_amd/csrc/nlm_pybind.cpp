@@ -259,6 +259,30 @@ PYBIND11_MODULE(nonlinearmedium, m) {
            },
            "Gradients (dL/d envelope, dL/d poling profile) of the last greensForward for dL/dC, dL/dS "
            "(convention dL/dRe + i dL/dIm) (extension; Chi2PDC).", "gC"_a, "gS"_a);
+  base.def("batchForward",
+           [](Medium& md, const carr& in, bool inTimeDomain, int ckptEvery) {
+             Grid g = to_grid(in);
+             Grid r;
+             {
+               py::gil_scoped_release nogil;
+               r = md.batchForward(g, inTimeDomain, ckptEvery);
+             }
+             return owned(std::move(r));
+           },
+           "batchSignalSimulation that keeps field checkpoints on the device for batchAdjoint "
+           "(extension; Chi3GNLSE).", "inputProfs"_a, "inTimeDomain"_a = false, "ckptEvery"_a = 0);
+  base.def("batchAdjoint",
+           [](Medium& md, const carr& gOut) {
+             Grid g = to_grid(gOut);
+             Grid r;
+             {
+               py::gil_scoped_release nogil;
+               r = md.batchAdjoint(g);
+             }
+             return owned(std::move(r));
+           },
+           "dL/d(inputProfs) of the last batchForward for dL/d(outputs) (convention dL/dRe + i dL/dIm; "
+           "extension; Chi3GNLSE).", "gOut"_a);
   base.def("enableTiming", &Medium::enableTiming, "Record HIP-event kernel times of each run (extension).", "on"_a);
   base.def("setStepTable", &Medium::setStepTable,
            "One-mode undepleted media: propagate with precomputed RK4 step propagators (default) or evaluate "

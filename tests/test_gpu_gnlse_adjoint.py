@@ -1,0 +1,104 @@
+"""Differentiable Chi3GNLSE batch (batchForward / batchAdjoint) on the device
+against torch's autograd of the Chi3GNLSE restatement (tests/gnlse_torch.py,
+itself checked against the oracle), at nFreqs handled by the single-transform
+kernels (P = 1) and by the staged transforms (16384, 32768).
+
+Gradients follow g = dL/dRe + i dL/dIm.  Tolerance: relative L2 <= 1e-10."""
+import math
+
+import numpy as np
+import pytest
+
+import gnlse_torch as GT
+from conftest import rel_l2
+from test_gnlse_torch_reference import sech_inputs, small_kwargs
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(nlm, oracle, N, nZ, rows, in_time, tMax, length):
+    kw = small_kwargs(N, nZ, length)
+    kw["tMax"] = tMax
+    X = sech_inputs(N, rows, N + nZ, tMax)
+    if not in_time:
+        X = np.fft.fft(X, axis=1)
+    ref = oracle.Chi3GNLSE(**kw)
+    t = GT.GnlseTables(kw, ref.omega, ref.ramanResponse)
+    return kw, X, t
+
+
+@pytest.mark.parametrize("N,nZ,rows,in_time,ckpt,tMax,length", [
+    (128, 12, 2, True, 3, 10.0, 0.02),
+    (256, 20, 3, False, 0, 10.0, 0.02),
+    (64, 9, 1, True, 1, 10.0, 0.02),
+    (1024, 7, 2, False, 4, 20.0, 0.01),
+    (16384, 5, 2, True, 2, 50.0, 0.002),
+    (32768, 4, 1, False, 0, 50.0, 0.002),
+])
+def test_adjoint_matches_autograd(nlm, oracle, N, nZ, rows, in_time, ckpt, tMax, length):
+    print("gnlse adjoint N=%d nZ=%d rows=%d in_time=%s ckpt=%d" % (N, nZ, rows, in_time, ckpt), flush=True)
+    kw, X, t = _case(nlm, oracle, N, nZ, rows, in_time, tMax, length)
+    w, c = GT.loss_weights(rows, N, N + 1)
+    Yref, gref = GT.reference_gradient(X, t, in_time, w, c)
+    dev = nlm.Chi3GNLSE(**kw)
+    Y = dev.batchForward(X, inTimeDomain=in_time, ckptEvery=ckpt)
+    assert rel_l2(Y, Yref) < 1e-10
+    # the checkpointing forward equals the plain batch
+    assert rel_l2(Y, dev.batchSignalSimulation(X, inTimeDomain=in_time)) < 1e-10
+    g = dev.batchAdjoint(GT.grad_out(Y, w, c))
+    assert np.isfinite(g).all()
+    assert rel_l2(g, gref) < 1e-10
+
+
+def test_adjoint_is_linear_in_output_gradient(nlm, oracle):
+    N, nZ, rows = 256, 15, 2
+    kw, X, t = _case(nlm, oracle, N, nZ, rows, True, 10.0, 0.02)
+    dev = nlm.Chi3GNLSE(**kw)
+    dev.batchForward(X, inTimeDomain=True, ckptEvery=4)
+    r = np.random.default_rng(5)
+    g1 = r.standard_normal((rows, N)) + 1j * r.standard_normal((rows, N))
+    g2 = r.standard_normal((rows, N)) + 1j * r.standard_normal((rows, N))
+    a, b = 0.7, -1.3
+    lhs = dev.batchAdjoint(a * g1 + b * g2)
+    rhs = a * dev.batchAdjoint(g1) + b * dev.batchAdjoint(g2)
+    assert rel_l2(lhs, rhs) < 1e-12
+
+
+def test_adjoint_matches_device_fd(nlm, oracle):
+    """Directional derivatives of the band-energy loss (spectral energy in
+    omega in [5, 15]) against central differences of the device forward."""
+    N, nZ, rows = 512, 30, 2
+    kw, X, t = _case(nlm, oracle, N, nZ, rows, True, 20.0, 0.05)
+    dev = nlm.Chi3GNLSE(**kw)
+    band = (np.asarray(dev.omega) >= 5) & (np.asarray(dev.omega) <= 15)
+
+    def L(x):
+        F = np.fft.fft(dev.batchSignalSimulation(x, inTimeDomain=True), axis=1)
+        return float(np.sum(np.abs(F[:, band]) ** 2))
+
+    Y = dev.batchForward(X, inTimeDomain=True)
+    F = np.fft.fft(Y, axis=1)
+    gF = 2 * F * band[None, :]
+    g = dev.batchAdjoint(N * np.fft.ifft(gF, axis=1))
+    r = np.random.default_rng(9)
+    eps = 1e-6
+    for _ in range(4):
+        d = r.standard_normal((rows, N)) + 1j * r.standard_normal((rows, N))
+        d /= np.linalg.norm(d)
+        fd = (L(X + eps * d) - L(X - eps * d)) / (2 * eps)
+        an = float(np.real(np.vdot(g, d)))
+        assert abs(fd - an) <= 1e-6 * np.linalg.norm(g)
+
+
+def test_adjoint_errors(nlm, oracle):
+    kw = small_kwargs(128, 10)
+    dev = nlm.Chi3GNLSE(**kw)
+    with pytest.raises(RuntimeError):
+        dev.batchAdjoint(np.zeros((1, 128), np.complex128))
+    X = sech_inputs(128, 2, 1, 10.0)
+    dev.batchForward(X, inTimeDomain=True)
+    with pytest.raises(ValueError):
+        dev.batchAdjoint(np.zeros((3, 128), np.complex128))
+    pdc = nlm.Chi2PDC(relativeLength=1.0, nlLength=1.0, beta2=1.0, beta2s=1.0, tPrecision=128, nZSteps=10)
+    with pytest.raises(ValueError):
+        pdc.batchForward(X, inTimeDomain=True)
