@@ -2,6 +2,7 @@
 // nlm_adjoint.hpp for the method): a forward launch that keeps checkpoints,
 // then the reverse sweep, the deterministic reductions and the adjoint of the
 // linear pump grid.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -17,11 +18,20 @@ __global__ void k_env_grad(const cplx* __restrict__ ifft, const cplx* __restrict
 
 int nlm_fft_device(const cplx* in, cplx* out, int n, int count, bool inverse, hipStream_t s);
 
-typedef void (*adj_fn)(const AParams&, int grid, hipStream_t);
+typedef void (*sweep_fn)(const AParams&, int grid, hipStream_t);
+typedef void (*env_fn)(const cplx*, int, const double*, double, const double*, const cplx*, cplx*, int grid,
+                       hipStream_t);
 
 template <int L>
-static void launch_adj(const AParams& ap, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_adj_rev<L>), dim3(grid), dim3(AdjLaunch<L>::THREADS), 0, s, ap);
+static void launch_sweep(const AParams& ap, int grid, hipStream_t s) {
+  constexpr int FA = AdjShape<L>::FA;
+  hipLaunchKernelGGL((k_adj_sweep<L, FA>), dim3(grid), dim3(AdjShape<L>::THREADS), 0, s, ap);
+}
+template <int L>
+static void launch_env(const cplx* gP, int nZP, const double* Dp, double dzp, const double* strength, const cplx* tw,
+                       cplx* out, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_adj_env_rows<L>), dim3(grid), dim3(FftShape<L>::T), 0, s, gP, nZP, Dp, dzp, strength, tw,
+                     out);
 }
 
 #define NLM_L_SWITCH2(FN, L) \
@@ -40,11 +50,23 @@ static void launch_adj(const AParams& ap, int grid, hipStream_t s) {
   }                           \
   return nullptr;
 
-static adj_fn pick_adj(int L) { NLM_L_SWITCH2(launch_adj, L) }
+static sweep_fn pick_sweep(int L) { NLM_L_SWITCH2(launch_sweep, L) }
+static env_fn pick_env(int L) { NLM_L_SWITCH2(launch_env, L) }
 
-static int adj_threads(int N) {
-  const int T = N / 8;
-  return T >= 256 ? T : 256;
+static int sweep_fields_per_wg(int L) {
+  switch (L) {
+    case 3: return AdjShape<3>::FA;
+    case 4: return AdjShape<4>::FA;
+    case 5: return AdjShape<5>::FA;
+    case 6: return AdjShape<6>::FA;
+    case 7: return AdjShape<7>::FA;
+    case 8: return AdjShape<8>::FA;
+    case 9: return AdjShape<9>::FA;
+    case 10: return AdjShape<10>::FA;
+    case 11: return AdjShape<11>::FA;
+    case 12: return AdjShape<12>::FA;
+    default: return AdjShape<13>::FA;
+  }
 }
 
 extern "C" int nlm_greens_forward(nlm_plan* p, int in_time, int n_in, const int32_t* inputs, int n_out,
@@ -111,26 +133,24 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
     return set_err(NLM_INVALID_ARGUMENT, "envelope gradient needs the linear pump simulation");
   const int N = p->N, L = p->L, nZ = p->nZ, nZP = 2 * nZ - 1;
   const size_t nF = 2 * (size_t)N, W = N, total = W * N;
-  const int TH = adj_threads(N);
-  const int FA = TH / (N / 8);
+  const int FA = sweep_fields_per_wg(L);
   const int nWG = (int)((nF + FA - 1) / FA);
-  const int nWaves = nWG * (TH / 64);
+  // steps per sweep launch: the workgroup partials of one launch stay within ~1 GiB
+  const size_t perStep = (size_t)nWG * 2 * N;
+  int chunk = (int)std::max<size_t>(1, ((size_t)1 << 30) / (perStep * sizeof(cplx)));
+  if (chunk > nZ - 1) chunk = nZ - 1 > 0 ? nZ - 1 : 1;
+  const bool rows = g_pump_time || g_envelope;
+  const int nEnv = std::min(nZP - 1 > 0 ? nZP - 1 : 1, 512);
   int e;
   if ((e = p->gC.ensure(total)) || (e = p->gS.ensure(total)) || (e = p->gG.ensure(nF * W)) ||
-      (e = p->gcoefPart.ensure((size_t)nWaves * nZ * 3)) || (e = p->gcoef.ensure((size_t)nZ * 3)) ||
-      (e = p->stepTab.ensure((size_t)nZ * N * 2)) || (e = p->stepInv.ensure((size_t)nZ * N * 2)))
+      (e = p->gcoefPart.ensure((size_t)nZ * 3 * N)) || (e = p->gcoef.ensure((size_t)nZ * 3)) ||
+      (e = p->stepTab.ensure((size_t)nZ * N * 2)) || (e = p->stepInv.ensure((size_t)nZ * N * 2)) ||
+      (e = p->gpumpPart.ensure(perStep * chunk)) || (e = p->gtmp.ensure(nF * 2 * N)))
     return e;
-  if (g_pump_time) {
-    if ((e = p->gpumpPart.ensure((size_t)nWG * nZP * N)) || (e = p->gpump.ensure((size_t)nZP * N))) return e;
-  }
-  if (g_envelope) {
-    if ((e = p->gF0Part.ensure((size_t)nWG * 3 * N)) || (e = p->gF0.ensure(2 * (size_t)N)) ||
-        (e = p->gtmp.ensure(4 * (size_t)N)))
-      return e;
-  }
+  if (rows && (e = p->gpump.ensure((size_t)nZP * N))) return e;
+  if (g_envelope && ((e = p->gF0Part.ensure((size_t)nEnv * N)) || (e = p->gF0.ensure(3 * (size_t)N)))) return e;
   HIP_TRY(hipMemcpyAsync(p->gC.p, gC, total * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
   HIP_TRY(hipMemcpyAsync(p->gS.p, gS, total * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
-  HIP_TRY(hipMemsetAsync(p->gcoefPart.p, 0, (size_t)nWaves * nZ * 3 * sizeof(cplx), p->stream));
   hipLaunchKernelGGL(k_greens_assemble_adjoint, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, p->stream,
                      p->gC.p, p->gS.p, p->gG.p, N, 1, 1, 1, (const cplx*)nullptr);
   // the states between checkpoints are rebuilt with the inverse step propagators
@@ -143,66 +163,62 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
   ap.nFields = (int)nF;
   ap.K = p->ckptEvery;
   ap.nSeg = p->nSeg;
-  ap.inTime = p->gInTime;
   ap.outTime = p->gInTime;
-  ap.nOut = 1;
-  ap.outPos[0] = 0;
-  ap.outPos[1] = ap.outPos[2] = ap.outPos[3] = -1;
+  ap.chunk = chunk;
   ap.ckpt = p->ckpt.p;
   ap.fout = p->G.p;
-  ap.inv = p->stepInv.p;
   ap.gout = p->gG.p;
-  ap.gin = nullptr;
-  ap.gcoef = p->gcoefPart.p;
-  ap.gpump = g_pump_time ? p->gpumpPart.p : nullptr;
-  if (g_envelope) {
-    ap.genvF = p->gF0Part.p;                   // [nWG][2][N]
-    ap.genvT = p->gF0Part.p + (size_t)nWG * 2 * N;  // [nWG][N]
-    ap.Dp = p->Dp.p;
-    ap.ph1 = p->gtmp.p + 2 * N;
-    ap.ph2 = p->gtmp.p + 3 * N;
-    ap.dzp = p->dzp;
-    ap.strength = p->hasStrength ? p->strength.p : nullptr;
-    hipLaunchKernelGGL(k_pump_phase_steps, dim3((N + 255) / 256), dim3(256), 0, p->stream, p->Dp.p, p->dzp, N,
-                       p->gtmp.p + 2 * N, p->gtmp.p + 3 * N);
-  }
+  ap.state = p->gtmp.p;
+  ap.part = p->gpumpPart.p;
+  ap.tab = p->stepTab.p;
+  ap.inv = p->stepInv.p;
   ap.tw = p->tw;
   ap.dstep = p->dstep.p;
-  ap.halfIn = p->halfIn.p;
   ap.halfOut = p->halfOut.p;
-  ap.coef = p->coef.p;
-  ap.pump0 = p->pumpT[0].p;
-  ap.tab = p->stepTab.p;
   tic(p, 2);
-  pick_adj(L)(ap, nWG, p->stream);
+  if (nZ > 1) {  // gm of step 0 is never read (it has no RK stages)
+    const sweep_fn sweep = pick_sweep(L);
+    for (int iHi = nZ - 1; iHi >= 1; iHi -= chunk) {
+      ap.iHi = iHi;
+      ap.iLo = iHi - chunk > 0 ? iHi - chunk : 0;
+      sweep(ap, nWG, p->stream);
+      const int count = ap.iHi - ap.iLo;
+      const unsigned blocks = (unsigned)(count * ((N + 63) / 64));
+      hipLaunchKernelGGL(k_adj_reduce, dim3(blocks), dim3(64 * RED_WAVES), 0, p->stream, ap.part, nWG, chunk, ap.iLo,
+                         N, p->coef.p, p->pumpT[0].p, p->gcoefPart.p);
+    }
+  }
   HIP_TRY(hipGetLastError());
   tic(p, 3);
-  const long nc = (long)nZ * 3;
-  hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, p->stream, p->gcoefPart.p, nc,
-                     nWaves, p->gcoef.p);
-  if (g_pump_time) {
+  hipLaunchKernelGGL(k_adj_coef, dim3((unsigned)nZ), dim3(256), 0, p->stream, p->gcoefPart.p, p->pumpT[0].p, N, nZ,
+                     p->gcoef.p);
+  if (rows) {
     const long np = (long)nZP * N;
-    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, p->stream, p->gpumpPart.p,
-                       np, nWG, p->gpump.p);
+    hipLaunchKernelGGL(k_adj_pump_rows, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, p->stream,
+                       p->gcoefPart.p, p->coef.p, N, nZ, p->gpump.p);
   }
   HIP_TRY(hipGetLastError());
   if (g_envelope) {
-    // g_F0 = sum of the workgroup partials; g_env = N IFFT(g_F0) + s_0 g_row0
-    hipLaunchKernelGGL(k_sum_partials, dim3((N + 255) / 256), dim3(256), 0, p->stream, ap.genvF, (long)N, 2 * nWG,
-                       p->gF0.p);
-    hipLaunchKernelGGL(k_sum_partials, dim3((N + 255) / 256), dim3(256), 0, p->stream, ap.genvT, (long)N, nWG,
-                       p->gF0.p + N);
+    // g_F0 = sum over rows r >= 1 (workgroup partials); g_env = N IFFT(g_F0) + s_0 g_P[0]
+    const double* strength = p->hasStrength ? p->strength.p : nullptr;
+    if (nZP > 1) {
+      pick_env(L)(p->gpump.p, nZP, p->Dp.p, p->dzp, strength, p->tw, p->gF0Part.p, nEnv, p->stream);
+      hipLaunchKernelGGL(k_sum_partials, dim3((N + 255) / 256), dim3(256), 0, p->stream, p->gF0Part.p, (long)N, nEnv,
+                         p->gF0.p);
+    } else {
+      HIP_TRY(hipMemsetAsync(p->gF0.p, 0, (size_t)N * sizeof(cplx), p->stream));
+    }
     HIP_TRY(hipGetLastError());
-    if ((e = nlm_fft_device(p->gF0.p, p->gtmp.p, N, 1, true, p->stream))) return e;
+    if ((e = nlm_fft_device(p->gF0.p, p->gF0.p + N, N, 1, true, p->stream))) return e;
     double s0 = 1.0;
     if (p->hasStrength) HIP_TRY(hipMemcpy(&s0, p->strength.p, sizeof(double), hipMemcpyDeviceToHost));
-    hipLaunchKernelGGL(k_env_grad, dim3((N + 255) / 256), dim3(256), 0, p->stream, p->gtmp.p, p->gF0.p + N, s0, N,
-                       p->gtmp.p + N);
+    hipLaunchKernelGGL(k_env_grad, dim3((N + 255) / 256), dim3(256), 0, p->stream, p->gF0.p + N, p->gpump.p, s0, N,
+                       p->gF0.p + 2 * N);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(g_envelope, p->gtmp.p + N, N * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipMemcpyAsync(g_envelope, p->gF0.p + 2 * N, N * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
   }
   tic(p, 4);
-  HIP_TRY(hipMemcpyAsync(g_coef, p->gcoef.p, nc * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipMemcpyAsync(g_coef, p->gcoef.p, (size_t)nZ * 3 * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
   if (g_pump_time)
     HIP_TRY(hipMemcpyAsync(g_pump_time, p->gpump.p, (size_t)nZP * N * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));

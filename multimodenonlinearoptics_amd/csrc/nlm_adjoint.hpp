@@ -7,19 +7,20 @@
 // (b = F a) has g_a = N * IFFT(g_b) and the scaled inverse DFT has
 // g_a = FFT(g_b) / N.
 //
-// The forward Green's launch (k_prop_tab with kp.ckpt set) stores the
-// time-domain field every K steps.  k_adj_rev sweeps all steps backwards,
-// rebuilding each forward state on chip by inverting the step (the states
-// between checkpoints are recomputed, backwards, instead of replayed from the
-// checkpoint and stored): adjoint of the dispersion step (forward FFT, conj(exp(i dz D)),
-// inverse FFT), then the transposed RK4 Jacobian of the DiffEq.  The per-step
-// coefficient gradients are reduced within each wave (deterministic shuffles)
-// and written as per-wave partials.  The pump enters through its linear
-// propagation P_r = s_r IFFT(FFT(envelope) exp(i r dzp D_p)), so the gradient
-// of each pump row, summed over the workgroup's fields in LDS, is transformed
-// and folded into a per-workgroup accumulator of dL/dFFT(envelope) right
-// away; nothing of size nZ x N is written per workgroup.  All partials are
-// summed by separate passes in a fixed order (no atomics).
+// The forward Green's launch (k_prop_tab with kp.ckpt set) runs each step as
+// its R-linear propagator r = alpha a + beta conj(a) (the RK4 step of the
+// medium, tabulated per step and time point) followed by the dispersion step
+// a' = IFFT(D FFT(r)), and stores the time-domain field every K steps.  The
+// reverse sweep (k_adj_sweep) walks the steps backwards for every field:
+//   h = g_r = IFFT(conj(D) FFT(g_a')),   g_a = conj(alpha) h + beta conj(h),
+// and the table gradients g_alpha = sum_fields conj(a) h, g_beta = sum_fields a h.
+// The forward states are rebuilt on chip by inverting the step (dispersion is
+// unitary, the R-linear step is invertible), anchored at the checkpoints.
+// Each workgroup sums g_alpha, g_beta over its fields in LDS and writes one
+// partial per step; a second kernel (k_adj_reduce) sums the partials in a
+// fixed order and applies the chain rule from (alpha, beta) = (R(1) -+ i R(i))/2
+// through the RK4 step to m_s = c_s P_s, the products of the step
+// coefficients and pump rows.  Nothing of size nZ x N is written per field.
 //
 // Forward map replaced: signalSimulationTemplate (src/_NonlinearMedium.hpp:160-239)
 // with Chi2PDC::DiffEq (solvers/Chi2PDC.cpp:27-47); assembly of
@@ -30,144 +31,114 @@
 
 struct AParams {
   int nZ, nFields, K, nSeg;
-  int inTime, outTime, nOut;
-  int outPos[4];
-  const cplx* ckpt;    // [nFields][nSeg][N] scaled time-domain states a_{sK}
-  const cplx* fout;    // [nFields][nOut*N] the forward's output rows, domain = outTime
-  const cplx* inv;     // [nZ][N][2] inverse RK4 step propagators
-  const cplx* gout;    // [nFields][nOut*N], domain = outTime
-  cplx* gin;           // [nFields][N] gradient w.r.t. the input (input domain) or nullptr
-  cplx* gcoef;         // [nWaves][nZ][3] per-wave partials of the step coefficients
-  cplx* gpump;         // [nWG][nZP][N] per-workgroup pump-row partials, or nullptr
-  cplx* genvF;         // [nWG][2][N] per-workgroup partials of dL/dFFT(envelope), or nullptr
-  cplx* genvT;         // [nWG][N] per-workgroup partials of dL/d(pump row 0) (with genvF)
-  const double* Dp;    // [N] pump dispersion D_p(omega)        (genvF)
-  const cplx* ph1;     // [N] exp(i dzp D_p), ph2 = exp(2 i dzp D_p) (genvF)
-  const cplx* ph2;
-  const double* strength;  // [2nZ-1] beam-profile factor per pump row, or nullptr
-  double dzp;
+  int outTime;
+  int iHi, iLo;        // this launch: steps iHi down to iLo + 1
+  int chunk;           // step capacity of `part`
+  const cplx* ckpt;    // [nFields][nSeg][N] time-domain states a_{sK}
+  const cplx* fout;    // [nFields][N] the forward's output rows, domain = outTime
+  const cplx* gout;    // [nFields][N] dL/d(output rows), domain = outTime
+  cplx* state;         // [nFields][2][N] a_{i-1} and g_{a_i} between launches
+  cplx* part;          // [nWG][chunk][2][N] workgroup sums of conj(a) h, a h
+  const cplx* tab;     // [nZ][N][2] RK4 step propagators (alpha, beta) / N
+  const cplx* inv;     // [nZ][N][2] inverse step propagators
   const cplx* tw;
   const cplx* dstep;
-  const cplx* halfIn;
   const cplx* halfOut;
-  const cplx* coef;    // [nZ][3] step coefficients c = pol * nlStep * mismatch
-  const cplx* pump0;   // [2nZ-1][N] pump time grid
-  const cplx* tab;     // [nZ][N][2] RK4 step propagators (alpha, beta) / N
 };
 
+// Fields per workgroup of the sweep: as many as keep the workgroup at 256
+// lanes (one wave per SIMD, so each wave has the whole register file for the
+// field, its gradient and the transform twiddles), at least 256 workgroups
+// for the 2N fields, and the LDS (one transform buffer per field) within 80 KB.
+// More fields per workgroup make the per-step partials smaller.
 template <int L>
-struct AdjLaunch {
-  static constexpr int T = FftShape<L>::T;
-  // 256 threads (one wave per SIMD) leave each wave the full register file:
-  // the adjoint holds the field, its gradient and three pump-row gradients
-  static constexpr int FA = T >= 256 ? 1 : 256 / T;  // fields per workgroup
+struct AdjShape {
+  static constexpr int N = 1 << L, T = FftShape<L>::T;
+  // two transform buffers per field (the gradient and the state are transformed together)
+  static constexpr int byLds = (80 * 1024) / (2 * FftShape<L>::lds_elems * 16);
+  static constexpr int byWG = N / 128 > 1 ? N / 128 : 1;
+  static constexpr int byLanes = T < 256 ? 256 / T : 1;
+  static constexpr int m1 = byLds < byWG ? byLds : byWG;
+  static constexpr int m2 = m1 < byLanes ? m1 : byLanes;
+  static constexpr int FA = m2 >= 4 ? 4 : m2 >= 2 ? 2 : 1;
   static constexpr int THREADS = FA * T;
-  static constexpr int WAVES = (THREADS + 63) / 64;
-  // LDS: one FFT buffer per field; with several fields also a second row
-  // buffer per field (the pump-row gradients are summed over the fields through
-  // the two) and the envelope-gradient accumulators and phases of groups 0, 1
-  static constexpr int LDS_ELEMS = (FA == 1 ? 1 : 2 * FA + 4) * FftShape<L>::lds_elems;
-  static constexpr int ACC_OFF = 2 * FA * FftShape<L>::lds_elems;
+  static constexpr bool FUSE = 2 * FA * FftShape<L>::lds_elems * 16 <= 160 * 1024 && THREADS <= 256;
+  static constexpr int BUFS = FUSE ? 2 * FA : FA;  // LDS transform buffers per workgroup
 };
 
-// wave-wide sum (xor butterfly; every lane ends with the total)
-NLM_DEV double wave_sum(double v) {
+// Sum conj(x) h (CONJ) or x h over the workgroup's FA fields and write the
+// total to dst[0..N) (through the fields' transform buffers, which are free
+// between transforms).
+template <int L, int FA, bool CONJ>
+NLM_DEV void wg_sum_product(const cplx (&x)[8], const cplx (&h)[8], bool active, cplx* lds, int lf, int t,
+                            cplx* __restrict__ dst) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T, E = F::lds_elems;
+  if constexpr (FA == 1) {
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+    for (int m = 0; m < 8; m++) dst[t + T * m] = active ? (CONJ ? conj(x[m]) : x[m]) * h[m] : mk(0, 0);
+  } else {
+    lds_barrier();
+#pragma unroll
+    for (int m = 0; m < 8; m++) lds[lf * E + lpad(t + T * m)] = active ? (CONJ ? conj(x[m]) : x[m]) * h[m] : mk(0, 0);
+    lds_barrier();
+#pragma unroll
+    for (int j = 0; j < 8 / FA; j++) {
+      const int k = lf * T + t + FA * T * j;
+      cplx s = lds[lpad(k)];
+#pragma unroll
+      for (int f = 1; f < FA; f++) s = s + lds[f * E + lpad(k)];
+      dst[k] = s;
+    }
+  }
 }
 
-// One forward split step i from the scaled state a (time domain) with the
-// step propagator table: a <- IFFT(FFT(alpha a + beta conj(a)) exp(i dz D)).
 template <int L>
-NLM_DEV void fwd_step_tab(cplx (&a)[8], int i, const AParams& ap, cplx* buf, int t, const FftTwiddles<L>& tw) {
-  constexpr int N = FftShape<L>::N, T = FftShape<L>::T;
-  const cplx* row = ap.tab + (size_t)i * N * 2;
+NLM_DEV void prop_row(cplx (&a)[8], cplx (&b)[8], const cplx* __restrict__ row, int t) {
+  constexpr int T = FftShape<L>::T;
 #pragma unroll
   for (int m = 0; m < 8; m++) {
-    const cplx al = row[(size_t)(t + T * m) * 2], be = row[(size_t)(t + T * m) * 2 + 1];
-    const cplx x = a[m] * (double)N;  // the table holds alpha / N, beta / N (exact scaling)
-    a[m] = mk(al.re * x.re - al.im * x.im + be.re * x.re + be.im * x.im,
-              al.re * x.im + al.im * x.re + be.im * x.re - be.re * x.im);
-  }
-  fft_field<L, false>(a, buf, t, tw);
-#pragma unroll
-  for (int m = 0; m < 8; m++) a[m] = a[m] * ap.dstep[t + T * m];
-  fft_field<L, true>(a, buf, t, tw);
-  scale_inv<L>(a);
-}
-
-// Sum a pump-row gradient over the workgroup's fields (through LDS) and write
-// it to this workgroup's partial row.
-template <int L>
-NLM_DEV void emit_pump_row(const cplx (&v)[8], cplx* lds, int lf, int t, cplx* dst) {
-  using F = FftShape<L>;
-  constexpr int N = F::N, T = F::T, FA = AdjLaunch<L>::FA, TH = AdjLaunch<L>::THREADS;
-  lds_barrier();
-#pragma unroll
-  for (int m = 0; m < 8; m++) lds[lf * F::lds_elems + lpad(t + T * m)] = v[m];
-  lds_barrier();
-  const int tau = lf * T + t;
-  for (int k = tau; k < N; k += TH) {
-    cplx s = lds[lpad(k)];
-    for (int f = 1; f < FA; f++) s = s + lds[f * F::lds_elems + lpad(k)];
-    dst[k] = s;
+    a[m] = row[(size_t)(t + T * m) * 2];
+    b[m] = row[(size_t)(t + T * m) * 2 + 1];
   }
 }
 
-// Reverse sweep of the Chi2PDC Green's function (k = c P conj(y), one signal
-// mode, one pump): one field per T lanes, FA fields per workgroup, all steps in
-// one launch.  The forward states are rebuilt backwards on chip: the
-// dispersion step is unitary and the RK4 step x -> alpha x + beta conj(x) of
-// this medium is invertible, so a_{i-1} = R_i^-1(IFFT(FFT(a_i) conj(exp(i dz D)))),
-// starting from the forward's output row and replaced by the stored
-// checkpoint at every multiple of K steps, which bounds the rounding drift.
-template <int L>
-__global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adj_rev(AParams ap) {
+// Reverse sweep over steps iHi .. iLo+1 for FA fields per workgroup (T lanes each).
+template <int L, int FA>
+__global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
   using F = FftShape<L>;
-  constexpr int N = F::N, T = F::T, FA = AdjLaunch<L>::FA;
-  __shared__ cplx lds[AdjLaunch<L>::LDS_ELEMS];
+  constexpr int N = F::N, T = F::T, E = F::lds_elems;
+  constexpr bool FUSE = AdjShape<L>::FUSE;
+  __shared__ cplx lds[AdjShape<L>::BUFS * E];
   const int lf = threadIdx.x / T, t = threadIdx.x % T;
   const int field = blockIdx.x * FA + lf;
   const bool active = field < ap.nFields;
   const int fld = active ? field : 0;
-  cplx* buf = lds + lf * F::lds_elems;
-  const int wave = (blockIdx.x * AdjLaunch<L>::THREADS + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  const int nZP = 2 * ap.nZ - 1;
+  cplx* buf = lds + lf * E;
 
   FftTwiddles<L> tw;
   fft_load_twiddles<L>(tw, ap.tw, t);
-  const int i1 = ap.nZ - 1;
-
-  // gradient of the last frequency row: output = F_last * halfOut (freq) or its IFFT (time)
-  cplx g[8];
-  cplx carry[8];  // pump row 2i gradient from step i+1 (its k1 stage)
-  const int op = ap.outPos[0];
+  cplx g[8], x[8];
+  bool gFreq;
+  if (ap.iHi == ap.nZ - 1) {
+    // the output row is halfOut * FFT(a') (frequency) or its IFFT (time)
 #pragma unroll
-  for (int m = 0; m < 8; m++)
-    g[m] = (active && op >= 0) ? ap.gout[(size_t)fld * ap.nOut * N + (size_t)op * N + t + T * m] : mk(0, 0);
-  if (ap.outTime) {  // output = IFFT(F) -> g_F = FFT(g) / N
-    fft_field<L, false>(g, buf, t, tw);
-    scale_inv<L>(g);
-  }
+    for (int m = 0; m < 8; m++) g[m] = active ? ap.gout[(size_t)fld * N + t + T * m] : mk(0, 0);
+    if (ap.outTime) {  // output = IFFT(F) -> g_F = FFT(g) / N
+      fft_field<L, false>(g, buf, t, tw);
+      scale_inv<L>(g);
+    }
 #pragma unroll
-  for (int m = 0; m < 8; m++) {
-    g[m] = conj(ap.halfOut[t + T * m]) * g[m];
-    carry[m] = mk(0, 0);
-  }
-  bool gFreq = true;  // g currently holds a frequency-domain gradient
-
-  // state a_{i-1} of the step being reversed, time domain
-  cplx x[8];
-  if (i1 >= 1) {
+    for (int m = 0; m < 8; m++) g[m] = conj(ap.halfOut[t + T * m]) * g[m];
+    gFreq = true;
+    const int i1 = ap.nZ - 1;
     if ((i1 - 1) % ap.K == 0) {
 #pragma unroll
       for (int m = 0; m < 8; m++) x[m] = ap.ckpt[((size_t)fld * ap.nSeg + (i1 - 1) / ap.K) * N + t + T * m];
     } else {
-      // F_last = output * conj(halfOut), r_{nZ-1} = IFFT(F_last conj(exp(i dz D))), a = R^-1(r)
+      // F_last = output * conj(halfOut), r = IFFT(F_last conj(exp(i dz D))), a = R^-1(r)
 #pragma unroll
-      for (int m = 0; m < 8; m++) x[m] = ap.fout[(size_t)fld * ap.nOut * N + (size_t)op * N + t + T * m];
+      for (int m = 0; m < 8; m++) x[m] = ap.fout[(size_t)fld * N + t + T * m];
       if (ap.outTime) fft_field<L, false>(x, buf, t, tw);
 #pragma unroll
       for (int m = 0; m < 8; m++) x[m] = conj(ap.halfOut[t + T * m] * ap.dstep[t + T * m]) * x[m];
@@ -180,250 +151,242 @@ __global__ void __launch_bounds__(AdjLaunch<L>::THREADS) k_adj_rev(AParams ap) {
         x[m] = ia * x[m] + ib * conj(x[m]);
       }
     }
-  }
-  cplx* gpumpWG = ap.gpump ? ap.gpump + (size_t)blockIdx.x * nZP * N : nullptr;
-
-  // envelope gradient accumulators: group 0 takes the even pump rows 2i,
-  // group 1 (group 0 if FA == 1) the odd rows 2i-1; ph = exp(-i r dzp D_p)
-  // for the group's next row r, advanced by exp(2 i dzp D_p) per step
-  // (registers when FA == 1, LDS otherwise)
-  constexpr int E = F::lds_elems;
-  const bool envAcc = ap.genvF != nullptr;
-  cplx acc[FA == 1 ? 8 : 1], ph[FA == 1 ? 8 : 1];
-  cplx* accL = lds + AdjLaunch<L>::ACC_OFF + (lf & 1) * 2 * E;
-  cplx* phL = accL + E;
-  if (envAcc && lf < 2) {
-    const int r0 = lf == 0 ? 2 * i1 : 2 * i1 - 1;
+  } else {
+    const cplx* st = ap.state + (size_t)fld * 2 * N;
 #pragma unroll
     for (int m = 0; m < 8; m++) {
-      double sn, cs;
-      sincos_reduced((double)r0 * ap.dzp * ap.Dp[t + T * m], &sn, &cs);
-      if constexpr (FA == 1) {
-        acc[m] = mk(0, 0);
-        ph[m] = mk(cs, -sn);
-      } else {
-        accL[lpad(t + T * m)] = mk(0, 0);
-        phL[lpad(t + T * m)] = mk(cs, -sn);
-      }
+      x[m] = st[t + T * m];
+      g[m] = st[N + t + T * m];
     }
+    gFreq = false;
   }
 
-  {
-    for (int i = i1; i >= 1; i--) {
-      // adjoint of the dispersion step: F = FFT(r) exp(i dz D); a_i = IFFT(F) (i < nZ-1)
+  cplx ds[8];  // conj(exp(i dz D)) / N
+#pragma unroll
+  for (int m = 0; m < 8; m++) ds[m] = conj(ap.dstep[t + T * m]) * (1.0 / N);
+  // this step's propagator (alpha, beta) / N and the inverse propagator of the
+  // step before it, loaded one step ahead so their latency hides behind the transforms
+  cplx al[8], be[8];
+  prop_row<L>(al, be, ap.tab + (size_t)ap.iHi * N * 2, t);
+
+  for (int i = ap.iHi; i > ap.iLo; i--) {
+    // adjoint of the dispersion step: h = g_r = IFFT(conj(exp(i dz D)) FFT(g));
+    // with FUSE the state's dispersion step is undone in the same transforms:
+    // xr = IFFT(conj(exp(i dz D)) FFT(a_{i-1})), from which a_{i-2} = R_{i-1}^-1(xr)
+    const bool rebuild = i >= 2 && (i - 2) % ap.K != 0;
+    const bool fused = FUSE && rebuild && !gFreq;
+    cplx xr[8];
+    if (fused) {
+#pragma unroll
+      for (int m = 0; m < 8; m++) xr[m] = x[m];
+      fft_field2<L, false>(g, xr, buf, lds + (FA + lf) * E, t, tw);
+#pragma unroll
+      for (int m = 0; m < 8; m++) {
+        g[m] = ds[m] * g[m];
+        xr[m] = ds[m] * xr[m];
+      }
+      fft_field2<L, true>(g, xr, buf, lds + (FA + lf) * E, t, tw);
+    } else {
       if (!gFreq) {
         fft_field<L, false>(g, buf, t, tw);
-        scale_inv<L>(g);
+      } else {
+#pragma unroll
+        for (int m = 0; m < 8; m++) g[m] = g[m] * (double)N;  // ds carries 1/N
       }
 #pragma unroll
-      for (int m = 0; m < 8; m++) g[m] = conj(ap.dstep[t + T * m]) * g[m];
-      fft_field<L, true>(g, buf, t, tw);  // g_r = N * IFFT(.) = unscaled inverse
-      gFreq = false;
+      for (int m = 0; m < 8; m++) g[m] = ds[m] * g[m];
+      fft_field<L, true>(g, buf, t, tw);  // unscaled inverse
+    }
+    gFreq = false;
 
-      // adjoint of the RK4 step (src/_NonlinearMedium.hpp:207-210 with
-      // Chi2PDC::DiffEq, solvers/Chi2PDC.cpp:27-47) at the stored state a_{i-1}:
-      // m_s = c_s P_s, k = m conj(y)  ->  g_y = m conj(g_k), g_m = y g_k
-      const cplx* c = ap.coef + (size_t)i * 3;
-      const cplx c0 = c[0], c1 = c[1], c2 = c[2];
-      cplx gc0 = mk(0, 0), gc1 = mk(0, 0), gc2 = mk(0, 0);
-      // pump-row gradients of this step: registers (FA == 1) or the field's two
-      // LDS row buffers, free once the transform above has finished
-      cplx rowE[FA == 1 ? 8 : 1], rowO[FA == 1 ? 8 : 1];
-      const bool rows = gpumpWG || envAcc;
-      if constexpr (FA > 1) {
-        if (rows) lds_barrier();
-      }
+    // adjoint of r = alpha x + beta conj(x), h = g: g_alpha += conj(x) h,
+    // g_beta += x h (workgroup sums), g_x = conj(alpha) h + beta conj(h)
+    cplx* dst = ap.part + (((size_t)blockIdx.x * ap.chunk + (i - ap.iLo - 1)) * 2) * N;
+    wg_sum_product<L, FA, true>(x, g, active, lds, lf, t, dst);
+    wg_sum_product<L, FA, false>(x, g, active, lds, lf, t, dst + N);
 #pragma unroll
-      for (int m = 0; m < 8; m++) {
-        const int k = t + T * m;
-        const cplx xs = x[m];
-        const cplx P0 = ap.pump0[(size_t)(2 * i - 2) * N + k];
-        const cplx P1 = ap.pump0[(size_t)(2 * i - 1) * N + k];
-        const cplx P2 = ap.pump0[(size_t)(2 * i) * N + k];
-        const cplx m0 = c0 * P0, m1 = c1 * P1, m2 = c2 * P2;
-        const cplx y2 = xs + 0.5 * (m0 * conj(xs));
-        const cplx y3 = xs + 0.5 * (m1 * conj(y2));
-        const cplx y4 = xs + m1 * conj(y3);
-        const cplx gr = g[m];
-        const cplx gk4 = gr * (1. / 6.);
-        cplx gk3 = gr * (1. / 3.), gk2 = gr * (1. / 3.), gk1 = gk4, gx = gr, gy;
-        gy = m2 * conj(gk4);
-        const cplx gm2 = y4 * gk4;
-        gx = gx + gy;
-        gk3 = gk3 + gy;
-        gy = m1 * conj(gk3);
-        cplx gm1 = y3 * gk3;
-        gx = gx + gy;
-        gk2 = gk2 + 0.5 * gy;
-        gy = m1 * conj(gk2);
-        gm1 = gm1 + y2 * gk2;
-        gx = gx + gy;
-        gk1 = gk1 + 0.5 * gy;
-        gy = m0 * conj(gk1);
-        const cplx gm0 = xs * gk1;
-        gx = gx + gy;
-        g[m] = active ? gx : mk(0, 0);
-        gc0 = gc0 + conj(P0) * gm0;
-        gc1 = gc1 + conj(P1) * gm1;
-        gc2 = gc2 + conj(P2) * gm2;
-        const cplx re = active ? conj(c2) * gm2 + carry[m] : mk(0, 0);
-        const cplx ro = active ? conj(c1) * gm1 : mk(0, 0);
-        carry[m] = active ? conj(c0) * gm0 : mk(0, 0);
-        if constexpr (FA == 1) {
-          rowE[m] = re;
-          rowO[m] = ro;
-        } else if (rows) {
-          lds[lf * E + lpad(k)] = re;
-          lds[(FA + lf) * E + lpad(k)] = ro;
+    for (int m = 0; m < 8; m++) {
+      const cplx h = g[m], a = al[m] * (double)N, b = be[m] * (double)N;
+      g[m] = conj(a) * h + b * conj(h);
+    }
+    if (i - 1 > ap.iLo) prop_row<L>(al, be, ap.tab + (size_t)(i - 1) * N * 2, t);
+
+    // state a_{i-2} for the next (earlier) step
+    if (i >= 2) {
+      if (!rebuild) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) x[m] = ap.ckpt[((size_t)fld * ap.nSeg + (i - 2) / ap.K) * N + t + T * m];
+      } else {
+        if (!fused) {
+#pragma unroll
+          for (int m = 0; m < 8; m++) xr[m] = x[m];
+          fft_field<L, false>(xr, buf, t, tw);
+#pragma unroll
+          for (int m = 0; m < 8; m++) xr[m] = ds[m] * xr[m];
+          fft_field<L, true>(xr, buf, t, tw);
         }
-      }
-      if constexpr (FA > 1) {
-        if (rows) lds_barrier();
-      }
-      if (!active) gc0 = gc1 = gc2 = mk(0, 0);
-      // per-wave partial of the coefficient gradients of step i
-      const double r0 = wave_sum(gc0.re), i0r = wave_sum(gc0.im), r1 = wave_sum(gc1.re), i1r = wave_sum(gc1.im),
-                   r2 = wave_sum(gc2.re), i2r = wave_sum(gc2.im);
-      if (lane == 0) {
-        cplx* dst = ap.gcoef + ((size_t)wave * ap.nZ + i) * 3;
-        dst[0] = mk(r0, i0r);
-        dst[1] = mk(r1, i1r);
-        dst[2] = mk(r2, i2r);
-      }
-      // pump rows 2i (k4 of step i + k1 of step i+1) and 2i-1 (k2, k3 of step i)
-      if (gpumpWG) {
-        cplx* dE = gpumpWG + (size_t)(2 * i) * N;
-        cplx* dO = gpumpWG + (size_t)(2 * i - 1) * N;
-        if constexpr (FA == 1) {
+        const cplx* iv = ap.inv + (size_t)(i - 1) * N * 2;
 #pragma unroll
-          for (int m = 0; m < 8; m++) {
-            dE[t + T * m] = rowE[m];
-            dO[t + T * m] = rowO[m];
-          }
-        } else {
-          for (int k = threadIdx.x; k < N; k += AdjLaunch<L>::THREADS) {
-            cplx se = mk(0, 0), so = mk(0, 0);
-            for (int f = 0; f < FA; f++) {
-              se = se + lds[f * E + lpad(k)];
-              so = so + lds[(FA + f) * E + lpad(k)];
-            }
-            dE[k] = se;
-            dO[k] = so;
-          }
-        }
-      }
-      if (envAcc) {
-        // g_F0 += s_r / N conj(exp(i r dzp D_p)) FFT(g_row r)   (rows r >= 1)
-        const double sE = (ap.strength ? ap.strength[2 * i] : 1.0) / N;
-        const double sO = (ap.strength ? ap.strength[2 * i - 1] : 1.0) / N;
-        if constexpr (FA == 1) {
-          fft_field<L, false>(rowE, buf, t, tw);
-          fft_field<L, false>(rowO, buf, t, tw);
-#pragma unroll
-          for (int m = 0; m < 8; m++) {
-            const cplx p1 = ap.ph1[t + T * m], p2 = ap.ph2[t + T * m];
-            const cplx phO = ph[m] * p1;
-            acc[m] = acc[m] + sE * (ph[m] * rowE[m]) + sO * (phO * rowO[m]);
-            ph[m] = ph[m] * p2;
-          }
-        } else {
-          // group 0 transforms the workgroup sum of row 2i, group 1 that of
-          // row 2i-1; the other groups transform zeros (uniform barriers)
-          cplx v[8];
-          const int base = lf == 0 ? 0 : FA * E;
-#pragma unroll
-          for (int m = 0; m < 8; m++) {
-            cplx s = mk(0, 0);
-            if (lf < 2)
-              for (int f = 0; f < FA; f++) s = s + lds[base + f * E + lpad(t + T * m)];
-            v[m] = s;
-          }
-          fft_field<L, false>(v, buf, t, tw);
-          if (lf < 2) {
-            const double sc = lf == 1 ? sO : sE;
-#pragma unroll
-            for (int m = 0; m < 8; m++) {
-              const int q = lpad(t + T * m);
-              const cplx phm = phL[q];
-              accL[q] = accL[q] + sc * (phm * v[m]);
-              phL[q] = phm * ap.ph2[t + T * m];
-            }
-          }
-        }
-      }
-      // state a_{i-2} for the next (earlier) step
-      if (i >= 2) {
-        if ((i - 2) % ap.K == 0) {
-#pragma unroll
-          for (int m = 0; m < 8; m++) x[m] = ap.ckpt[((size_t)fld * ap.nSeg + (i - 2) / ap.K) * N + t + T * m];
-        } else {
-          fft_field<L, false>(x, buf, t, tw);
-#pragma unroll
-          for (int m = 0; m < 8; m++) x[m] = conj(ap.dstep[t + T * m]) * x[m];
-          fft_field<L, true>(x, buf, t, tw);
-          scale_inv<L>(x);
-          const cplx* iv = ap.inv + (size_t)(i - 1) * N * 2;
-#pragma unroll
-          for (int m = 0; m < 8; m++) {
-            const cplx ia = iv[(size_t)(t + T * m) * 2], ib = iv[(size_t)(t + T * m) * 2 + 1];
-            x[m] = ia * x[m] + ib * conj(x[m]);
-          }
+        for (int m = 0; m < 8; m++) {
+          const cplx ia = iv[(size_t)(t + T * m) * 2], ib = iv[(size_t)(t + T * m) * 2 + 1];
+          x[m] = ia * xr[m] + ib * conj(xr[m]);
         }
       }
     }
   }
-  if (gpumpWG) emit_pump_row<L>(carry, lds, lf, t, gpumpWG);  // row 0
-  if (envAcc) {
-    // row 0 enters the envelope directly: workgroup sum of its gradient
-    if constexpr (FA == 1) {
+  if (ap.iLo > 0 && active) {
+    cplx* st = ap.state + (size_t)fld * 2 * N;
 #pragma unroll
-      for (int m = 0; m < 8; m++) {
-        ap.genvT[(size_t)blockIdx.x * N + t + T * m] = carry[m];
-        ap.genvF[((size_t)blockIdx.x * 2) * N + t + T * m] = acc[m];
-        ap.genvF[((size_t)blockIdx.x * 2 + 1) * N + t + T * m] = mk(0, 0);
-      }
-    } else {
-      lds_barrier();
-#pragma unroll
-      for (int m = 0; m < 8; m++) lds[lf * E + lpad(t + T * m)] = carry[m];
-      lds_barrier();
-      for (int k = threadIdx.x; k < N; k += AdjLaunch<L>::THREADS) {
-        cplx s = mk(0, 0);
-        for (int f = 0; f < FA; f++) s = s + lds[f * E + lpad(k)];
-        ap.genvT[(size_t)blockIdx.x * N + k] = s;
-      }
-      if (lf < 2) {
-#pragma unroll
-        for (int m = 0; m < 8; m++)
-          ap.genvF[((size_t)blockIdx.x * 2 + lf) * N + t + T * m] = accL[lpad(t + T * m)];
-      }
+    for (int m = 0; m < 8; m++) {
+      st[t + T * m] = x[m];
+      st[N + t + T * m] = g[m];
     }
-  }
-
-  // adjoint of the prologue: a_0 = IFFT(F0), F0 = in * halfIn (in = FFT(x) for time input)
-  if (!gFreq) {
-    fft_field<L, false>(g, buf, t, tw);
-    scale_inv<L>(g);
-  }
-#pragma unroll
-  for (int m = 0; m < 8; m++) g[m] = conj(ap.halfIn[t + T * m]) * g[m];
-  if (ap.inTime) fft_field<L, true>(g, buf, t, tw);
-  if (ap.gin && active) {
-#pragma unroll
-    for (int m = 0; m < 8; m++) ap.gin[(size_t)fld * N + t + T * m] = g[m];
   }
 }
 
-// exp(i dzp D_p), exp(2 i dzp D_p)
-__global__ void k_pump_phase_steps(const double* __restrict__ Dp, double dzp, int N, cplx* __restrict__ ph1,
-                                   cplx* __restrict__ ph2) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= N) return;
-  double sn, cs;
-  sincos(dzp * Dp[k], &sn, &cs);
-  ph1[k] = mk(cs, sn);
-  sincos(2.0 * dzp * Dp[k], &sn, &cs);
-  ph2[k] = mk(cs, sn);
+// Sum the partials of steps iLo+1 .. iLo+count and apply the chain rule from
+// (alpha, beta) to m_s = c_s P_s: R(x) = x + (k1 + 2 k2 + 2 k3 + k4) / 6 with
+// k_s = m_s conj(y_s) (Chi2PDC::DiffEq, solvers/Chi2PDC.cpp:27-47),
+// alpha = (R(1) - i R(i)) / 2, beta = (R(1) + i R(i)) / 2.  Writes gm[i][s][tau].
+// One block per (step, 64 time points): wave q sums workgroups
+// q, q + RED_WAVES, ... (fixed order), then lane tau adds the wave sums in order.
+constexpr int RED_WAVES = 8;
+__global__ void __launch_bounds__(64 * RED_WAVES) k_adj_reduce(const cplx* __restrict__ part, int nWG, int chunk,
+                                                               int iLo, int N, const cplx* __restrict__ coef,
+                                                               const cplx* __restrict__ pump, cplx* __restrict__ gm) {
+  __shared__ cplx red[2][RED_WAVES][64];
+  const int tiles = (N + 63) / 64;
+  const int jj = blockIdx.x / tiles, k0 = (blockIdx.x % tiles) * 64;
+  const int q = threadIdx.x / 64, lane = threadIdx.x % 64, k = k0 + lane;
+  cplx ga = mk(0, 0), gb = mk(0, 0);
+  if (k < N) {
+    for (int w = q; w < nWG; w += RED_WAVES) {
+      const cplx* p = part + (((size_t)w * chunk + jj) * 2) * N;
+      ga = ga + p[k];
+      gb = gb + p[N + k];
+    }
+  }
+  red[0][q][lane] = ga;
+  red[1][q][lane] = gb;
+  __syncthreads();
+  if (q != 0 || k >= N) return;
+  ga = red[0][0][lane];
+  gb = red[1][0][lane];
+#pragma unroll
+  for (int r = 1; r < RED_WAVES; r++) {
+    ga = ga + red[0][r][lane];
+    gb = gb + red[1][r][lane];
+  }
+  const int i = iLo + 1 + jj;
+  const cplx* c = coef + (size_t)i * 3;
+  const cplx m0 = c[0] * pump[(size_t)(2 * i - 2) * N + k];
+  const cplx m1 = c[1] * pump[(size_t)(2 * i - 1) * N + k];
+  const cplx m2 = c[2] * pump[(size_t)(2 * i) * N + k];
+  cplx gm0 = mk(0, 0), gm1 = mk(0, 0), gm2 = mk(0, 0);
+#pragma unroll
+  for (int pass = 0; pass < 2; pass++) {
+    // x = 1 carries g_R(1) = (g_alpha + g_beta) / 2, x = i carries g_R(i) = i (g_alpha - g_beta) / 2
+    const cplx x = pass == 0 ? mk(1, 0) : mk(0, 1);
+    const cplx gr = pass == 0 ? 0.5 * (ga + gb) : mk(0, 0.5) * (ga - gb);
+    const cplx y2 = x + 0.5 * (m0 * conj(x));
+    const cplx y3 = x + 0.5 * (m1 * conj(y2));
+    const cplx y4 = x + m1 * conj(y3);
+    const cplx gk4 = gr * (1. / 6.);
+    cplx gk3 = gr * (1. / 3.), gk2 = gr * (1. / 3.), gk1 = gk4;
+    gm2 = gm2 + y4 * gk4;
+    gk3 = gk3 + m2 * conj(gk4);
+    gm1 = gm1 + y3 * gk3;
+    gk2 = gk2 + 0.5 * (m1 * conj(gk3));
+    gm1 = gm1 + y2 * gk2;
+    gk1 = gk1 + 0.5 * (m1 * conj(gk2));
+    gm0 = gm0 + x * gk1;
+  }
+  cplx* o = gm + (size_t)i * 3 * N;
+  o[k] = gm0;
+  o[N + k] = gm1;
+  o[2 * N + k] = gm2;
+}
+
+// g_coef[i][s] = sum_tau conj(P_s) gm[i][s] (fixed-order tree; one block per step).
+__global__ void __launch_bounds__(256) k_adj_coef(const cplx* __restrict__ gm, const cplx* __restrict__ pump, int N,
+                                                  int nZ, cplx* __restrict__ gcoef) {
+  __shared__ cplx red[3][256];
+  const int i = blockIdx.x, t = threadIdx.x;
+  cplx s0 = mk(0, 0), s1 = mk(0, 0), s2 = mk(0, 0);
+  if (i >= 1) {
+    const cplx* g = gm + (size_t)i * 3 * N;
+    for (int k = t; k < N; k += 256) {
+      s0 = s0 + conj(pump[(size_t)(2 * i - 2) * N + k]) * g[k];
+      s1 = s1 + conj(pump[(size_t)(2 * i - 1) * N + k]) * g[N + k];
+      s2 = s2 + conj(pump[(size_t)(2 * i) * N + k]) * g[2 * N + k];
+    }
+  }
+  red[0][t] = s0;
+  red[1][t] = s1;
+  red[2][t] = s2;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if (t < h) {
+      red[0][t] = red[0][t] + red[0][t + h];
+      red[1][t] = red[1][t] + red[1][t + h];
+      red[2][t] = red[2][t] + red[2][t + h];
+    }
+    __syncthreads();
+  }
+  if (t < 3) gcoef[(size_t)i * 3 + t] = red[t][0];
+}
+
+// Pump-row gradients g_P[r] = conj(c) gm: row 2j from k4 of step j and k1 of
+// step j+1, row 2j-1 from k2, k3 of step j.
+__global__ void k_adj_pump_rows(const cplx* __restrict__ gm, const cplx* __restrict__ coef, int N, int nZ,
+                                cplx* __restrict__ gP) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nZP = 2 * nZ - 1;
+  if (idx >= (long)nZP * N) return;
+  const int r = (int)(idx / N), k = (int)(idx % N);
+  cplx v = mk(0, 0);
+  if (r % 2 == 0) {
+    const int j = r / 2;
+    if (j >= 1) v = v + conj(coef[(size_t)j * 3 + 2]) * gm[((size_t)j * 3 + 2) * N + k];
+    if (j + 1 <= nZ - 1) v = v + conj(coef[(size_t)(j + 1) * 3]) * gm[((size_t)(j + 1) * 3) * N + k];
+  } else {
+    const int j = (r + 1) / 2;
+    v = conj(coef[(size_t)j * 3 + 1]) * gm[((size_t)j * 3 + 1) * N + k];
+  }
+  gP[idx] = v;
+}
+
+// Per-workgroup partial of g_F0 = sum_{r >= 1} s_r / N conj(exp(i r dzp D_p)) FFT(g_P[r])
+// over rows r = 1 + blockIdx.x, 1 + blockIdx.x + gridDim.x, ... (one field-transform per workgroup).
+template <int L>
+__global__ void __launch_bounds__(FftShape<L>::T) k_adj_env_rows(const cplx* __restrict__ gP, int nZP,
+                                                                 const double* __restrict__ Dp, double dzp,
+                                                                 const double* __restrict__ strength,
+                                                                 const cplx* __restrict__ twt, cplx* __restrict__ out) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T;
+  __shared__ cplx lds[F::lds_elems];
+  const int t = threadIdx.x;
+  FftTwiddles<L> tw;
+  fft_load_twiddles<L>(tw, twt, t);
+  cplx acc[8];
+#pragma unroll
+  for (int m = 0; m < 8; m++) acc[m] = mk(0, 0);
+  for (int r = 1 + blockIdx.x; r < nZP; r += gridDim.x) {
+    cplx v[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) v[m] = gP[(size_t)r * N + t + T * m];
+    fft_field<L, false>(v, lds, t, tw);
+    const double s = (strength ? strength[r] : 1.0) / N;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      double sn, cs;
+      sincos_reduced((double)r * dzp * Dp[t + T * m], &sn, &cs);
+      acc[m] = acc[m] + s * (mk(cs, -sn) * v[m]);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 8; m++) out[(size_t)blockIdx.x * N + t + T * m] = acc[m];
 }
 
 // Adjoint of the Green's assembly: gradients of the final rows G(e_j), G(i e_j)

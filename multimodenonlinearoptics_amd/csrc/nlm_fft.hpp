@@ -220,3 +220,84 @@ __device__ __forceinline__ void fft_field(cplx (&v)[8], cplx* __restrict__ buf, 
     fft_stages<L, INV, 1>(v, buf, t, tw);
   }
 }
+
+// Two fields transformed together (same direction, separate LDS buffers): the
+// exchanges of both share each barrier, so every lane has twice the independent
+// arithmetic between two waits.  Same layouts and results as fft_field.
+template <int L, bool INV, int s>
+__device__ __forceinline__ void fft_stages2(cplx (&v)[8], cplx (&u)[8], cplx* __restrict__ bv, cplx* __restrict__ bu,
+                                            int t, const FftTwiddles<L>& tw) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T, S = F::S;
+  constexpr int R = F::radix(s), P = F::before(s), Mp = N / P, M = Mp / R, NB = 8 / R;
+  lds_barrier();
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    const int beta = t + T * j;
+    const int b = beta % M, g = beta / M;
+#pragma unroll
+    for (int a = 0; a < R; a++) {
+      v[j * R + a] = bv[lpad(g * Mp + a * M + b)];
+      u[j * R + a] = bu[lpad(g * Mp + a * M + b)];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    dft_small<R, INV>(v + j * R);
+    dft_small<R, INV>(u + j * R);
+  }
+  if constexpr (s == S - 1) {
+    if constexpr (NB > 1) {
+      cplx o[8], q[8];
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+          o[j + NB * c] = v[j * R + c];
+          q[j + NB * c] = u[j * R + c];
+        }
+#pragma unroll
+      for (int m = 0; m < 8; m++) {
+        v[m] = o[m];
+        u[m] = q[m];
+      }
+    }
+  } else {
+    lds_barrier();
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const int beta = t + T * j;
+      const int b = beta % M, g = beta / M;
+#pragma unroll
+      for (int c = 0; c < R; c++) {
+        const cplx w = tw.w[s][j * R + c];
+        bv[lpad((g + P * c) * M + b)] = c == 0 ? v[j * R] : twmul<INV>(v[j * R + c], w);
+        bu[lpad((g + P * c) * M + b)] = c == 0 ? u[j * R] : twmul<INV>(u[j * R + c], w);
+      }
+    }
+    fft_stages2<L, INV, s + 1>(v, u, bv, bu, t, tw);
+  }
+}
+
+template <int L, bool INV>
+__device__ __forceinline__ void fft_field2(cplx (&v)[8], cplx (&u)[8], cplx* __restrict__ bv, cplx* __restrict__ bu,
+                                           int t, const FftTwiddles<L>& tw) {
+  using F = FftShape<L>;
+  constexpr int T = F::T, S = F::S;
+  dft_small<8, INV>(v);
+  dft_small<8, INV>(u);
+  if constexpr (S > 1) {
+#pragma unroll
+    for (int c = 1; c < 8; c++) {
+      v[c] = twmul<INV>(v[c], tw.w[0][c]);
+      u[c] = twmul<INV>(u[c], tw.w[0][c]);
+    }
+    lds_barrier();
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      bv[lpad(c * T + t)] = v[c];
+      bu[lpad(c * T + t)] = u[c];
+    }
+    fft_stages2<L, INV, 1>(v, u, bv, bu, t, tw);
+  }
+}
