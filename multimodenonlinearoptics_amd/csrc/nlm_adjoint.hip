@@ -144,7 +144,7 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
   int e;
   if ((e = p->gC.ensure(total)) || (e = p->gS.ensure(total)) || (e = p->gG.ensure(nF * W)) ||
       (e = p->gcoefPart.ensure((size_t)nZ * 3 * N)) || (e = p->gcoef.ensure((size_t)nZ * 3)) ||
-      (e = p->stepTab.ensure((size_t)nZ * N * 2)) || (e = p->stepInv.ensure((size_t)nZ * N * 2)) ||
+      (e = p->stepTab.ensure((size_t)nZ * N * 2)) ||
       (e = p->gpumpPart.ensure(perStep * chunk)) || (e = p->gtmp.ensure(nF * 2 * N)))
     return e;
   if (rows && (e = p->gpump.ensure((size_t)nZP * N))) return e;
@@ -153,9 +153,8 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
   HIP_TRY(hipMemcpyAsync(p->gS.p, gS, total * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
   hipLaunchKernelGGL(k_greens_assemble_adjoint, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, p->stream,
                      p->gC.p, p->gS.p, p->gG.p, N, 1, 1, 1, (const cplx*)nullptr);
-  // the states between checkpoints are rebuilt with the inverse step propagators
-  nlm_launch_step_table(NLM_CHI2PDC, p->coef.p, p->cconst.p, p->pumpT[0].p, N, nZ, p->stepTab.p, p->stream,
-                        p->stepInv.p);
+  // the step propagators (the states between checkpoints are rebuilt by inverting them)
+  nlm_launch_step_table(NLM_CHI2PDC, p->coef.p, p->cconst.p, p->pumpT[0].p, N, nZ, p->stepTab.p, p->stream);
   HIP_TRY(hipGetLastError());
   AParams ap;
   memset(&ap, 0, sizeof ap);
@@ -171,7 +170,6 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
   ap.state = p->gtmp.p;
   ap.part = p->gpumpPart.p;
   ap.tab = p->stepTab.p;
-  ap.inv = p->stepInv.p;
   ap.tw = p->tw;
   ap.dstep = p->dstep.p;
   ap.halfOut = p->halfOut.p;

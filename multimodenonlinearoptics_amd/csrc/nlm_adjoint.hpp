@@ -15,7 +15,8 @@
 //   h = g_r = IFFT(conj(D) FFT(g_a')),   g_a = conj(alpha) h + beta conj(h),
 // and the table gradients g_alpha = sum_fields conj(a) h, g_beta = sum_fields a h.
 // The forward states are rebuilt on chip by inverting the step (dispersion is
-// unitary, the R-linear step is invertible), anchored at the checkpoints.
+// unitary, the R-linear step is invertible: a = (conj(alpha) r - beta conj(r)) /
+// (|alpha|^2 - |beta|^2)), anchored at the checkpoints.
 // Each workgroup sums g_alpha, g_beta over its fields in LDS and writes one
 // partial per step; a second kernel (k_adj_reduce) sums the partials in a
 // fixed order and applies the chain rule from (alpha, beta) = (R(1) -+ i R(i))/2
@@ -40,7 +41,6 @@ struct AParams {
   cplx* state;         // [nFields][2][N] a_{i-1} and g_{a_i} between launches
   cplx* part;          // [nWG][chunk][2][N] workgroup sums of conj(a) h, a h
   const cplx* tab;     // [nZ][N][2] RK4 step propagators (alpha, beta) / N
-  const cplx* inv;     // [nZ][N][2] inverse step propagators
   const cplx* tw;
   const cplx* dstep;
   const cplx* halfOut;
@@ -64,6 +64,14 @@ struct AdjShape {
   static constexpr int THREADS = FA * T;
   static constexpr bool FUSE = 2 * FA * FftShape<L>::lds_elems * 16 <= 160 * 1024 && THREADS <= 256;
   static constexpr int BUFS = FUSE ? 2 * FA : FA;  // LDS transform buffers per workgroup
+  static constexpr int LDS_ELEMS = BUFS * FftShape<L>::lds_elems;
+};
+
+// Register twiddles behind an accessor tw(stage, entry) (fft_field1 / fft_field2).
+template <int L>
+struct RegTwiddles {
+  FftTwiddles<L> w;
+  __device__ __forceinline__ cplx operator()(int s, int idx) const { return w.w[s][idx]; }
 };
 
 // Sum conj(x) h (CONJ) or x h over the workgroup's FA fields and write the
@@ -103,21 +111,33 @@ NLM_DEV void prop_row(cplx (&a)[8], cplx (&b)[8], const cplx* __restrict__ row, 
   }
 }
 
+// x = R^-1(r) for the step r = alpha x + beta conj(x), given (alpha, beta) / N:
+// x = (conj(alpha) r - beta conj(r)) / (|alpha|^2 - |beta|^2).
+template <int L>
+NLM_DEV void inverse_step(cplx (&x)[8], const cplx (&r)[8], const cplx (&a)[8], const cplx (&b)[8]) {
+  constexpr double N = FftShape<L>::N;
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    const double den = N * (abs2(a[m]) - abs2(b[m]));
+    x[m] = (conj(a[m]) * r[m] - b[m] * conj(r[m])) * (1.0 / den);
+  }
+}
+
 // Reverse sweep over steps iHi .. iLo+1 for FA fields per workgroup (T lanes each).
 template <int L, int FA>
 __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T, E = F::lds_elems;
   constexpr bool FUSE = AdjShape<L>::FUSE;
-  __shared__ cplx lds[AdjShape<L>::BUFS * E];
+  __shared__ cplx lds[AdjShape<L>::LDS_ELEMS];
   const int lf = threadIdx.x / T, t = threadIdx.x % T;
   const int field = blockIdx.x * FA + lf;
   const bool active = field < ap.nFields;
   const int fld = active ? field : 0;
   cplx* buf = lds + lf * E;
 
-  FftTwiddles<L> tw;
-  fft_load_twiddles<L>(tw, ap.tw, t);
+  RegTwiddles<L> tw;
+  fft_load_twiddles<L>(tw.w, ap.tw, t);
   cplx g[8], x[8];
   bool gFreq;
   if (ap.iHi == ap.nZ - 1) {
@@ -125,7 +145,7 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
 #pragma unroll
     for (int m = 0; m < 8; m++) g[m] = active ? ap.gout[(size_t)fld * N + t + T * m] : mk(0, 0);
     if (ap.outTime) {  // output = IFFT(F) -> g_F = FFT(g) / N
-      fft_field<L, false>(g, buf, t, tw);
+      fft_field1<L, false>(g, buf, t, tw);
       scale_inv<L>(g);
     }
 #pragma unroll
@@ -139,17 +159,16 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
       // F_last = output * conj(halfOut), r = IFFT(F_last conj(exp(i dz D))), a = R^-1(r)
 #pragma unroll
       for (int m = 0; m < 8; m++) x[m] = ap.fout[(size_t)fld * N + t + T * m];
-      if (ap.outTime) fft_field<L, false>(x, buf, t, tw);
+      if (ap.outTime) fft_field1<L, false>(x, buf, t, tw);
 #pragma unroll
       for (int m = 0; m < 8; m++) x[m] = conj(ap.halfOut[t + T * m] * ap.dstep[t + T * m]) * x[m];
-      fft_field<L, true>(x, buf, t, tw);
+      fft_field1<L, true>(x, buf, t, tw);
       scale_inv<L>(x);
-      const cplx* iv = ap.inv + (size_t)i1 * N * 2;
+      cplx ra[8], rb[8], r[8];
+      prop_row<L>(ra, rb, ap.tab + (size_t)i1 * N * 2, t);
 #pragma unroll
-      for (int m = 0; m < 8; m++) {
-        const cplx ia = iv[(size_t)(t + T * m) * 2], ib = iv[(size_t)(t + T * m) * 2 + 1];
-        x[m] = ia * x[m] + ib * conj(x[m]);
-      }
+      for (int m = 0; m < 8; m++) r[m] = x[m];
+      inverse_step<L>(x, r, ra, rb);
     }
   } else {
     const cplx* st = ap.state + (size_t)fld * 2 * N;
@@ -164,10 +183,12 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
   cplx ds[8];  // conj(exp(i dz D)) / N
 #pragma unroll
   for (int m = 0; m < 8; m++) ds[m] = conj(ap.dstep[t + T * m]) * (1.0 / N);
-  // this step's propagator (alpha, beta) / N and the inverse propagator of the
-  // step before it, loaded one step ahead so their latency hides behind the transforms
-  cplx al[8], be[8];
+  // propagator rows (alpha, beta) / N of this step (al, be) and of the step
+  // before it (pa, pb, which also inverts it), loaded a step ahead so their
+  // latency hides behind the transforms
+  cplx al[8], be[8], pa[8], pb[8];
   prop_row<L>(al, be, ap.tab + (size_t)ap.iHi * N * 2, t);
+  if (ap.iHi >= 2) prop_row<L>(pa, pb, ap.tab + (size_t)(ap.iHi - 1) * N * 2, t);
 
   for (int i = ap.iHi; i > ap.iLo; i--) {
     // adjoint of the dispersion step: h = g_r = IFFT(conj(exp(i dz D)) FFT(g));
@@ -188,14 +209,14 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
       fft_field2<L, true>(g, xr, buf, lds + (FA + lf) * E, t, tw);
     } else {
       if (!gFreq) {
-        fft_field<L, false>(g, buf, t, tw);
+        fft_field1<L, false>(g, buf, t, tw);
       } else {
 #pragma unroll
         for (int m = 0; m < 8; m++) g[m] = g[m] * (double)N;  // ds carries 1/N
       }
 #pragma unroll
       for (int m = 0; m < 8; m++) g[m] = ds[m] * g[m];
-      fft_field<L, true>(g, buf, t, tw);  // unscaled inverse
+      fft_field1<L, true>(g, buf, t, tw);  // unscaled inverse
     }
     gFreq = false;
 
@@ -209,7 +230,6 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
       const cplx h = g[m], a = al[m] * (double)N, b = be[m] * (double)N;
       g[m] = conj(a) * h + b * conj(h);
     }
-    if (i - 1 > ap.iLo) prop_row<L>(al, be, ap.tab + (size_t)(i - 1) * N * 2, t);
 
     // state a_{i-2} for the next (earlier) step
     if (i >= 2) {
@@ -220,18 +240,19 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
         if (!fused) {
 #pragma unroll
           for (int m = 0; m < 8; m++) xr[m] = x[m];
-          fft_field<L, false>(xr, buf, t, tw);
+          fft_field1<L, false>(xr, buf, t, tw);
 #pragma unroll
           for (int m = 0; m < 8; m++) xr[m] = ds[m] * xr[m];
-          fft_field<L, true>(xr, buf, t, tw);
+          fft_field1<L, true>(xr, buf, t, tw);
         }
-        const cplx* iv = ap.inv + (size_t)(i - 1) * N * 2;
-#pragma unroll
-        for (int m = 0; m < 8; m++) {
-          const cplx ia = iv[(size_t)(t + T * m) * 2], ib = iv[(size_t)(t + T * m) * 2 + 1];
-          x[m] = ia * xr[m] + ib * conj(xr[m]);
-        }
+        inverse_step<L>(x, xr, pa, pb);
       }
+#pragma unroll
+      for (int m = 0; m < 8; m++) {
+        al[m] = pa[m];
+        be[m] = pb[m];
+      }
+      if (i >= 3) prop_row<L>(pa, pb, ap.tab + (size_t)(i - 2) * N * 2, t);
     }
   }
   if (ap.iLo > 0 && active) {

@@ -432,7 +432,7 @@ extern "C" int nlm_plan_destroy(nlm_plan* p) {
                           &p->G, &p->C, &p->S, &p->in, &p->out, &p->env, &p->F0, &p->dsp, &p->relI};
   for (auto* b : bufs) b->release();
   DevBuf<cplx>* abufs[] = {&p->ckpt, &p->gG, &p->gC, &p->gS, &p->gcoefPart, &p->gcoef, &p->gpumpPart, &p->gpump,
-                           &p->gF0Part, &p->gF0, &p->gtmp, &p->scale, &p->stepTab, &p->stepInv};
+                           &p->gF0Part, &p->gF0, &p->gtmp, &p->scale, &p->stepTab};
   for (auto* b : abufs) b->release();
   if (p->bigExec) (void)hipGraphExecDestroy(p->bigExec);
   p->bigState.release();
@@ -442,6 +442,8 @@ extern "C" int nlm_plan_destroy(nlm_plan* p) {
   p->strength.release();
   for (auto& ev : p->ev)
     if (ev) (void)hipEventDestroy(ev);
+  p->adjWork.release();
+  p->adjSeg.release();
   if (p->own) (void)hipStreamDestroy(p->own);
   delete p;
   return NLM_OK;

@@ -223,10 +223,11 @@ __device__ __forceinline__ void fft_field(cplx (&v)[8], cplx* __restrict__ buf, 
 
 // Two fields transformed together (same direction, separate LDS buffers): the
 // exchanges of both share each barrier, so every lane has twice the independent
-// arithmetic between two waits.  Same layouts and results as fft_field.
-template <int L, bool INV, int s>
+// arithmetic between two waits.  Same layouts and results as fft_field; the
+// twiddles come from an accessor tw(s, idx) returning FftTwiddles::w[s][idx].
+template <int L, bool INV, int s, class TW>
 __device__ __forceinline__ void fft_stages2(cplx (&v)[8], cplx (&u)[8], cplx* __restrict__ bv, cplx* __restrict__ bu,
-                                            int t, const FftTwiddles<L>& tw) {
+                                            int t, const TW& tw) {
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T, S = F::S;
   constexpr int R = F::radix(s), P = F::before(s), Mp = N / P, M = Mp / R, NB = 8 / R;
@@ -270,18 +271,23 @@ __device__ __forceinline__ void fft_stages2(cplx (&v)[8], cplx (&u)[8], cplx* __
       const int b = beta % M, g = beta / M;
 #pragma unroll
       for (int c = 0; c < R; c++) {
-        const cplx w = tw.w[s][j * R + c];
-        bv[lpad((g + P * c) * M + b)] = c == 0 ? v[j * R] : twmul<INV>(v[j * R + c], w);
-        bu[lpad((g + P * c) * M + b)] = c == 0 ? u[j * R] : twmul<INV>(u[j * R + c], w);
+        if (c == 0) {
+          bv[lpad(g * M + b)] = v[j * R];
+          bu[lpad(g * M + b)] = u[j * R];
+        } else {
+          const cplx w = tw(s, j * R + c);
+          bv[lpad((g + P * c) * M + b)] = twmul<INV>(v[j * R + c], w);
+          bu[lpad((g + P * c) * M + b)] = twmul<INV>(u[j * R + c], w);
+        }
       }
     }
     fft_stages2<L, INV, s + 1>(v, u, bv, bu, t, tw);
   }
 }
 
-template <int L, bool INV>
+template <int L, bool INV, class TW>
 __device__ __forceinline__ void fft_field2(cplx (&v)[8], cplx (&u)[8], cplx* __restrict__ bv, cplx* __restrict__ bu,
-                                           int t, const FftTwiddles<L>& tw) {
+                                           int t, const TW& tw) {
   using F = FftShape<L>;
   constexpr int T = F::T, S = F::S;
   dft_small<8, INV>(v);
@@ -289,8 +295,9 @@ __device__ __forceinline__ void fft_field2(cplx (&v)[8], cplx (&u)[8], cplx* __r
   if constexpr (S > 1) {
 #pragma unroll
     for (int c = 1; c < 8; c++) {
-      v[c] = twmul<INV>(v[c], tw.w[0][c]);
-      u[c] = twmul<INV>(u[c], tw.w[0][c]);
+      const cplx w = tw(0, c);
+      v[c] = twmul<INV>(v[c], w);
+      u[c] = twmul<INV>(u[c], w);
     }
     lds_barrier();
 #pragma unroll
@@ -299,5 +306,60 @@ __device__ __forceinline__ void fft_field2(cplx (&v)[8], cplx (&u)[8], cplx* __r
       bu[lpad(c * T + t)] = u[c];
     }
     fft_stages2<L, INV, 1>(v, u, bv, bu, t, tw);
+  }
+}
+
+// One field with twiddles from tw(s, idx) (see fft_field).
+template <int L, bool INV, int s, class TW>
+__device__ __forceinline__ void fft_stages1(cplx (&v)[8], cplx* __restrict__ buf, int t, const TW& tw) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T, S = F::S;
+  constexpr int R = F::radix(s), P = F::before(s), Mp = N / P, M = Mp / R, NB = 8 / R;
+  lds_barrier();
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    const int beta = t + T * j;
+    const int b = beta % M, g = beta / M;
+#pragma unroll
+    for (int a = 0; a < R; a++) v[j * R + a] = buf[lpad(g * Mp + a * M + b)];
+  }
+#pragma unroll
+  for (int j = 0; j < NB; j++) dft_small<R, INV>(v + j * R);
+  if constexpr (s == S - 1) {
+    if constexpr (NB > 1) {
+      cplx o[8];
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+#pragma unroll
+        for (int c = 0; c < R; c++) o[j + NB * c] = v[j * R + c];
+#pragma unroll
+      for (int m = 0; m < 8; m++) v[m] = o[m];
+    }
+  } else {
+    lds_barrier();
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const int beta = t + T * j;
+      const int b = beta % M, g = beta / M;
+#pragma unroll
+      for (int c = 0; c < R; c++)
+        buf[lpad((g + P * c) * M + b)] = c == 0 ? v[j * R] : twmul<INV>(v[j * R + c], tw(s, j * R + c));
+    }
+    fft_stages1<L, INV, s + 1>(v, buf, t, tw);
+  }
+}
+
+template <int L, bool INV, class TW>
+__device__ __forceinline__ void fft_field1(cplx (&v)[8], cplx* __restrict__ buf, int t, const TW& tw) {
+  using F = FftShape<L>;
+  constexpr int T = F::T, S = F::S;
+  dft_small<8, INV>(v);
+  if constexpr (S > 1) {
+#pragma unroll
+    for (int c = 1; c < 8; c++) v[c] = twmul<INV>(v[c], tw(0, c));
+    lds_barrier();
+#pragma unroll
+    for (int c = 0; c < 8; c++) buf[lpad(c * T + t)] = v[c];
+    fft_stages1<L, INV, 1>(v, buf, t, tw);
   }
 }

@@ -65,7 +65,6 @@ struct nlm_plan_s {
   double dzp = 0;
   DevBuf<cplx> ckpt, gG, gC, gS, gcoefPart, gcoef, gpumpPart, gpump, gF0Part, gF0, gtmp;
   DevBuf<cplx> scale;
-  DevBuf<cplx> stepInv;  // inverse RK4 step propagators (adjoint)
   // staged transform for Chi3GNLSE with nFreqs > 8192 (nlm_big.hip)
   DevBuf<cplx> bigState, bigH;
   DevBuf<int> bigBase;

@@ -466,4 +466,4 @@ prop_fn nlm_pick_prop(int model, int L);
 // Step-propagator kernel (one signal mode) for log2 N = L, and its table builder.
 prop_fn nlm_pick_prop_tab(int L);
 bool nlm_launch_step_table(int model, const cplx* coef, const cplx* cconst, const cplx* pump, int N, int nZ,
-                           cplx* tab, hipStream_t s, cplx* inv = nullptr);
+                           cplx* tab, hipStream_t s);

@@ -30,13 +30,10 @@ __device__ __forceinline__ cplx rk4_step(const cplx* c, const cplx* cc, cplx x, 
   return x + (((k1[0] + 2.0 * k2[0]) + 2.0 * k3[0]) + k4[0]) * (1. / 6.);
 }
 
-// tab[(i*N + k)*2 + {0,1}] = {alpha, beta} * scale for steps i = 1..nZ-1 and,
-// if inv is given, the inverse map x = a r + b conj(r):
-// inv[(i*N + k)*2 + {0,1}] = {conj(alpha), -beta} / (|alpha|^2 - |beta|^2)
+// tab[(i*N + k)*2 + {0,1}] = {alpha, beta} * scale for steps i = 1..nZ-1.
 template <class M>
 __global__ void k_step_table(const cplx* __restrict__ coef, const cplx* __restrict__ cconst,
-                             const cplx* __restrict__ pump, int N, int nZ, double scale, cplx* __restrict__ tab,
-                             cplx* __restrict__ inv) {
+                             const cplx* __restrict__ pump, int N, int nZ, double scale, cplx* __restrict__ tab) {
   static_assert(M::NS == 1 && M::NP == 1, "single signal mode, single pump");
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long total = (long)(nZ - 1) * N;
@@ -52,11 +49,6 @@ __global__ void k_step_table(const cplx* __restrict__ coef, const cplx* __restri
   const cplx al = 0.5 * (r1 - iri), be = 0.5 * (r1 + iri);
   tab[((size_t)i * N + k) * 2] = scale * al;
   tab[((size_t)i * N + k) * 2 + 1] = scale * be;
-  if (inv) {
-    const double det = abs2(al) - abs2(be);
-    inv[((size_t)i * N + k) * 2] = (1.0 / det) * conj(al);
-    inv[((size_t)i * N + k) * 2 + 1] = (-1.0 / det) * be;
-  }
 }
 
 template <int L>
@@ -157,18 +149,17 @@ prop_fn nlm_pick_prop_tab(int L) {
 }
 
 // Build the step propagator table of `model` (NLM_CHI3 or NLM_CHI2PDC) into tab
-// ([nZ][N][2] complex; row 0 unused) and, if inv is not null, the inverse
-// propagators into inv (same layout).  Returns false for other models.
+// ([nZ][N][2] complex; row 0 unused).  Returns false for other models.
 bool nlm_launch_step_table(int model, const cplx* coef, const cplx* cconst, const cplx* pump, int N, int nZ,
-                           cplx* tab, hipStream_t s, cplx* inv) {
+                           cplx* tab, hipStream_t s) {
   const long total = (long)(nZ - 1) * N;
   if (total <= 0) return true;
   const dim3 grid((unsigned)((total + 255) / 256)), block(256);
   const double scale = 1.0 / N;
   if (model == NLM_CHI2PDC)
-    hipLaunchKernelGGL(k_step_table<MChi2PDC>, grid, block, 0, s, coef, cconst, pump, N, nZ, scale, tab, inv);
+    hipLaunchKernelGGL(k_step_table<MChi2PDC>, grid, block, 0, s, coef, cconst, pump, N, nZ, scale, tab);
   else if (model == NLM_CHI3)
-    hipLaunchKernelGGL(k_step_table<MChi3>, grid, block, 0, s, coef, cconst, pump, N, nZ, scale, tab, inv);
+    hipLaunchKernelGGL(k_step_table<MChi3>, grid, block, 0, s, coef, cconst, pump, N, nZ, scale, tab);
   else
     return false;
   return true;
