@@ -1,4 +1,4 @@
-"""PyTorch autograd over the MI355X engine's adjoint Green's function.
+"""PyTorch autograd over the MI355X engine's adjoints.
 
 ``GreensFunction.apply(medium, envelope, poling)`` returns the Green's
 matrices (C, S) of a single-mode Chi2PDC medium as complex128 tensors, for a
@@ -10,6 +10,11 @@ dL/d envelope and dL/d poling in torch's complex-gradient convention.
 The forward replaces computeGreensFunction (src/_NonlinearMedium.cpp:263-375)
 for a Chi2PDC medium (solvers/Chi2PDC.cpp:27-47) after setPump(customPump)
 (src/_NonlinearMedium.cpp:207-222) and a per-step poling.
+
+``BatchSignal.apply(medium, inputs, inTimeDomain)`` is batchSignalSimulation of a
+Chi3GNLSE medium (src/_FullyNonlinearMedium.hpp:8-11, solvers/Chi3GNLSE.cpp:39-92)
+for a [rows, nFreqs] complex128 tensor; its backward runs the device adjoint
+(checkpointed forward, segment recomputation) and returns dL/d inputs.
 """
 import numpy as np
 import torch
@@ -29,7 +34,6 @@ class GreensFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gC, gS):
-        N = ctx.medium.nZSteps  # noqa: F841  (kept for clarity of the shapes below)
         gC = np.zeros(0) if gC is None else gC.detach().cpu().numpy()
         gS = np.zeros(0) if gS is None else gS.detach().cpu().numpy()
         n = ctx.medium.signalFreq.shape[1]
@@ -39,6 +43,28 @@ class GreensFunction(torch.autograd.Function):
             gS = np.zeros((n, n), dtype=np.complex128)
         g_env, g_pol = ctx.medium.greensAdjoint(gC.astype(np.complex128), gS.astype(np.complex128))
         return (None, torch.from_numpy(g_env).to(ctx.device), torch.from_numpy(g_pol).to(ctx.device), None)
+
+
+class BatchSignal(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, medium, inputs, inTimeDomain=True, ckpt_every=0):
+        x = inputs.detach().cpu().numpy().astype(np.complex128)
+        out = medium.batchForward(x, bool(inTimeDomain), int(ckpt_every))
+        ctx.medium = medium
+        ctx.device = inputs.device
+        return torch.from_numpy(out).to(inputs.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        gin = ctx.medium.batchAdjoint(g.detach().cpu().numpy().astype(np.complex128))
+        return None, torch.from_numpy(gin).to(ctx.device), None, None
+
+
+def band_energy_loss(out_time, omega, lo=5.0, hi=15.0):
+    """Spectral energy of the output rows (time domain) in lo <= omega <= hi."""
+    F = torch.fft.fft(out_time, dim=-1)
+    band = torch.as_tensor((np.asarray(omega) >= lo) & (np.asarray(omega) <= hi), device=out_time.device)
+    return (F.abs() ** 2 * band).sum()
 
 
 def squeezing_loss(S, r=1.5):

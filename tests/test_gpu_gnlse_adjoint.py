@@ -102,3 +102,16 @@ def test_adjoint_errors(nlm, oracle):
     pdc = nlm.Chi2PDC(relativeLength=1.0, nlLength=1.0, beta2=1.0, beta2s=1.0, tPrecision=128, nZSteps=10)
     with pytest.raises(ValueError):
         pdc.batchForward(X, inTimeDomain=True)
+
+
+def test_autograd_batch_signal_matches_reference(nlm, oracle):
+    import torch
+    from multimodenonlinearoptics_amd.autograd import BatchSignal, band_energy_loss
+    N, nZ, rows = 256, 16, 2
+    kw, X, t = _case(nlm, oracle, N, nZ, rows, True, 20.0, 0.05)
+    dev = nlm.Chi3GNLSE(**kw)
+    x = torch.tensor(X, requires_grad=True)
+    band_energy_loss(BatchSignal.apply(dev, x, True), dev.omega).backward()
+    xr = torch.tensor(X, requires_grad=True)
+    band_energy_loss(GT.forward(xr, t, True), dev.omega).backward()
+    assert rel_l2(x.grad.numpy(), xr.grad.numpy()) < 1e-10
