@@ -293,6 +293,51 @@ def gnlse_grad_benchmark(nlm, rows=64, nz=20000, n=16384, fwd_ms=None, fd_dirs=2
     }
 
 
+def compose_benchmark(n=512, reps=20):
+    """Green's composition of two media (Cascade::computeGreensFunction,
+    src/Cascade.cpp:128-129) at n x n (two 512-point Chi2PDC stages): the FP64
+    matrix-core kernel on device buffers, timed with HIP events on the stream it
+    runs on."""
+    import ctypes as C
+    import numpy as np
+    from multimodenonlinearoptics_amd import capi
+    L, H = capi.lib(), capi.hip()
+    r = np.random.default_rng(20261202 + 7)
+    host = [r.standard_normal((n, n)) + 1j * r.standard_normal((n, n)) for _ in range(4)]
+    dev = [capi.DeviceArray(a) for a in host]
+    out = [capi.DeviceArray(shape=(n, n)) for _ in range(2)]
+    stream, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    H.hipStreamCreate(C.byref(stream))
+    H.hipEventCreate(C.byref(e0))
+    H.hipEventCreate(C.byref(e1))
+    args = [d.ptr for d in dev + out] + [stream]
+
+    def run():
+        if L.nlm_greens_compose_dev(n, n, n, *args) != 0:
+            raise RuntimeError(L.nlm_last_error().decode())
+
+    run()
+    H.hipStreamSynchronize(stream)
+    H.hipEventRecord(e0, stream)
+    for _ in range(reps):
+        run()
+    H.hipEventRecord(e1, stream)
+    H.hipEventSynchronize(e1)
+    ms_total = C.c_float()
+    H.hipEventElapsedTime(C.byref(ms_total), e0, e1)
+    ms = ms_total.value / reps
+    H.hipEventDestroy(e0)
+    H.hipEventDestroy(e1)
+    H.hipStreamDestroy(stream)
+    C2, S2, C1, S1 = host
+    ref = C2 @ C1 + S2 @ S1.conj()
+    err = float(np.linalg.norm(out[0].numpy() - ref) / np.linalg.norm(ref))
+    flops = 32.0 * n ** 3  # two complex products per output matrix: 4 x 8 n^3
+    return {"workload": "C = C2 C1 + S2 conj(S1), S = C2 S1 + S2 conj(C1), %d x %d complex" % (n, n),
+            "kernel_ms": ms, "tflops": flops / (ms * 1e-3) / 1e12,
+            "frac_fp64_peak": flops / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, "rel_err": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -305,7 +350,8 @@ def main():
     ap.add_argument("--hetero-count", type=int, default=2000)
     ap.add_argument("--no-gnlse", action="store_true", help="skip the nFreqs=16384 Chi3GNLSE measurement")
     ap.add_argument("--no-gnlse-grad", action="store_true", help="skip the Chi3GNLSE input-gradient measurement")
-    ap.add_argument("--only", choices=["hetero", "gnlse", "grad", "gnlse_grad"], default=None,
+    ap.add_argument("--no-compose", action="store_true", help="skip the Green's composition measurement")
+    ap.add_argument("--only", choices=["hetero", "gnlse", "grad", "gnlse_grad", "compose"], default=None,
                     help="run only this section and print its record (for profiling)")
     ap.add_argument("--gnlse-nz", type=int, default=20000,
                     help="nZSteps of the Chi3GNLSE section (the medium is shortened in proportion)")
@@ -340,6 +386,8 @@ def main():
             out = gnlse_benchmark(nlm, nz=args.gnlse_nz)
         elif args.only == "gnlse_grad":
             out = gnlse_grad_benchmark(nlm, nz=args.gnlse_nz)
+        elif args.only == "compose":
+            out = compose_benchmark()
         else:
             out = grad_benchmark(nlm)
         print(json.dumps(out))
@@ -422,6 +470,9 @@ def main():
     if not args.no_gnlse_grad:
         rec["gnlse_grad"] = gnlse_grad_benchmark(nlm, nz=args.gnlse_nz,
                                                  fwd_ms=rec["gnlse"]["device_ms"] if "gnlse" in rec else None)
+
+    if not args.no_compose:
+        rec["compose"] = compose_benchmark()
 
     if not args.no_grad:
         g = grad_benchmark(nlm)

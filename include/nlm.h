@@ -222,6 +222,10 @@ int nlm_batch_adjoint(nlm_plan* plan, const double* g_out, double* g_in);
  * C2, S2: M x K; C1, S1: K x Ncol; C, S: M x Ncol (host, row-major). */
 int nlm_greens_compose(int M, int K, int Ncol, const double* C2, const double* S2, const double* C1,
                        const double* S1, double* C, double* S);
+/* The same on device buffers (complex, interleaved) on `stream` (a hipStream_t;
+ * NULL = the null stream), asynchronous.  Runs on the FP64 matrix cores. */
+int nlm_greens_compose_dev(int M, int K, int Ncol, const double* C2, const double* S2, const double* C1,
+                           const double* S1, double* C, double* S, void* stream);
 
 /* Standalone transform with the reference's conventions (src/CustomEigenFFT.h):
  * count rows of length n, forward unscaled / inverse scaled by 1/n. */

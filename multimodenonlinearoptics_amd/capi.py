@@ -18,6 +18,7 @@ SYMBOLS = [
     "nlm_plan_sync", "nlm_last_error", "nlm_version", "nlm_greens_forward", "nlm_greens_adjoint",
     "nlm_multi_create", "nlm_multi_launch", "nlm_multi_sync", "nlm_multi_output", "nlm_multi_classes",
     "nlm_multi_grid", "nlm_multi_destroy", "nlm_batch_forward", "nlm_batch_adjoint",
+    "nlm_greens_compose_dev",
 ]
 
 _lib = None
@@ -56,6 +57,7 @@ def lib():
         L.nlm_batch.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, vp]
         L.nlm_batch_dev.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, vp]
         L.nlm_greens_compose.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.nlm_greens_compose_dev.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -77,3 +79,61 @@ def fft(x, inverse=False):
     out = np.empty_like(x)
     check(lib().nlm_fft(n, x.size // n, x.ctypes.data, out.ctypes.data, int(inverse)))
     return out
+
+
+_hip = None
+
+
+def hip():
+    """The HIP runtime libnlm_hip.so is linked against (device buffers, streams
+    and events for callers of the *_dev entry points that hold no framework
+    tensors)."""
+    global _hip
+    if _hip is None:
+        lib()
+        H = C.CDLL("libamdhip64.so.7")
+        vp = C.c_void_p
+        H.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
+        H.hipFree.argtypes = [vp]
+        H.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+        H.hipStreamCreate.argtypes = [C.POINTER(vp)]
+        H.hipStreamDestroy.argtypes = [vp]
+        H.hipStreamSynchronize.argtypes = [vp]
+        H.hipEventCreate.argtypes = [C.POINTER(vp)]
+        H.hipEventDestroy.argtypes = [vp]
+        H.hipEventRecord.argtypes = [vp, vp]
+        H.hipEventSynchronize.argtypes = [vp]
+        H.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
+        _hip = H
+    return _hip
+
+
+def _hip_check(rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed with HIP error %d" % (what, rc))
+
+
+class DeviceArray:
+    """A device copy of a complex128 numpy array (freed with the object)."""
+
+    def __init__(self, a=None, shape=None):
+        import numpy as np
+        H = hip()
+        self.host = np.ascontiguousarray(a, dtype=np.complex128) if a is not None else np.zeros(shape, np.complex128)
+        self.ptr = C.c_void_p()
+        n = max(self.host.nbytes, 16)
+        _hip_check(H.hipMalloc(C.byref(self.ptr), n), "hipMalloc")
+        if a is not None and self.host.nbytes:
+            _hip_check(H.hipMemcpy(self.ptr, self.host.ctypes.data, self.host.nbytes, 1), "hipMemcpy")
+
+    def numpy(self):
+        import numpy as np
+        out = np.empty_like(self.host)
+        if out.nbytes:
+            _hip_check(hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2), "hipMemcpy")
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value and _hip is not None:
+            _hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()

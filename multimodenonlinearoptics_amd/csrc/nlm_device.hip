@@ -838,69 +838,3 @@ extern "C" int nlm_fft(int n, int count, const double* in, double* out, int inve
   return NLM_OK;
 }
 
-// ---------------------------------------------------------------------------
-// Cascade composition (src/Cascade.cpp:128-129):
-//   C = C2 C1 + S2 conj(S1),  S = C2 S1 + S2 conj(C1)
-// C2, S2: M x K; C1, S1: K x Ncol; LDS-tiled complex FP64 products.
-// ---------------------------------------------------------------------------
-
-static constexpr int CT = 16;
-
-__global__ void __launch_bounds__(CT * CT) k_compose(const cplx* __restrict__ C2, const cplx* __restrict__ S2,
-                                                      const cplx* __restrict__ C1, const cplx* __restrict__ S1,
-                                                      cplx* __restrict__ C, cplx* __restrict__ S, int M, int K,
-                                                      int Nc) {
-  __shared__ cplx a1[CT][CT + 1], a2[CT][CT + 1], b1[CT][CT + 1], b2[CT][CT + 1];
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int row = blockIdx.y * CT + ty, col = blockIdx.x * CT + tx;
-  cplx accC = mk(0, 0), accS = mk(0, 0);
-  for (int k0 = 0; k0 < K; k0 += CT) {
-    const int ka = k0 + tx, kb = k0 + ty;
-    a1[ty][tx] = (row < M && ka < K) ? C2[(size_t)row * K + ka] : mk(0, 0);
-    a2[ty][tx] = (row < M && ka < K) ? S2[(size_t)row * K + ka] : mk(0, 0);
-    b1[ty][tx] = (kb < K && col < Nc) ? C1[(size_t)kb * Nc + col] : mk(0, 0);
-    b2[ty][tx] = (kb < K && col < Nc) ? S1[(size_t)kb * Nc + col] : mk(0, 0);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < CT; k++) {
-      const cplx x1 = a1[ty][k], x2 = a2[ty][k], y1 = b1[k][tx], y2 = b2[k][tx];
-      accC = accC + x1 * y1 + x2 * conj(y2);
-      accS = accS + x1 * y2 + x2 * conj(y1);
-    }
-    __syncthreads();
-  }
-  if (row < M && col < Nc) {
-    C[(size_t)row * Nc + col] = accC;
-    S[(size_t)row * Nc + col] = accS;
-  }
-}
-
-extern "C" int nlm_greens_compose(int M, int K, int Nc, const double* C2, const double* S2, const double* C1,
-                                  const double* S1, double* C, double* S) {
-  if (M < 0 || K < 0 || Nc < 0 || !C2 || !S2 || !C1 || !S1 || !C || !S)
-    return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
-  if (M == 0 || Nc == 0) return NLM_OK;
-  DevBuf<cplx> dC2, dS2, dC1, dS1, dC, dS;
-  const size_t a = (size_t)M * K, b = (size_t)K * Nc, c = (size_t)M * Nc;
-  int e = NLM_OK;
-  if ((e = dC2.ensure(a)) || (e = dS2.ensure(a)) || (e = dC1.ensure(b)) || (e = dS1.ensure(b)) || (e = dC.ensure(c)) ||
-      (e = dS.ensure(c)))
-    return e;
-  hipError_t err = hipSuccess;
-  auto h2d = [&](cplx* d, const double* h, size_t n) {
-    if (err == hipSuccess && n) err = hipMemcpy(d, h, n * sizeof(cplx), hipMemcpyHostToDevice);
-  };
-  h2d(dC2.p, C2, a);
-  h2d(dS2.p, S2, a);
-  h2d(dC1.p, C1, b);
-  h2d(dS1.p, S1, b);
-  if (err == hipSuccess) {
-    dim3 grid((Nc + CT - 1) / CT, (M + CT - 1) / CT), block(CT, CT);
-    hipLaunchKernelGGL(k_compose, grid, block, 0, 0, dC2.p, dS2.p, dC1.p, dS1.p, dC.p, dS.p, M, K, Nc);
-    err = hipGetLastError();
-  }
-  if (err == hipSuccess) err = hipMemcpy(C, dC.p, c * sizeof(cplx), hipMemcpyDeviceToHost);
-  if (err == hipSuccess) err = hipMemcpy(S, dS.p, c * sizeof(cplx), hipMemcpyDeviceToHost);
-  if (err != hipSuccess) return set_err(NLM_DEVICE_ERROR, std::string("nlm_greens_compose: ") + hipGetErrorString(err));
-  return NLM_OK;
-}
