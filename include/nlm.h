@@ -227,6 +227,12 @@ int nlm_greens_compose(int M, int K, int Ncol, const double* C2, const double* S
 int nlm_greens_compose_dev(int M, int K, int Ncol, const double* C2, const double* S2, const double* C1,
                            const double* S1, double* C, double* S, void* stream);
 
+/* Cascade::computeGreensFunction (src/Cascade.cpp:110-135) over n media whose
+ * Green's functions were computed with nlm_greens(..., C = S = NULL) and are
+ * still on the device: composed in order (plans[0] first) on the device, the
+ * result (rows of plans[n-1]) x (cols of plans[0]) written to C, S (host). */
+int nlm_greens_cascade(int n, nlm_plan* const* plans, double* C, double* S);
+
 /* Standalone transform with the reference's conventions (src/CustomEigenFFT.h):
  * count rows of length n, forward unscaled / inverse scaled by 1/n. */
 int nlm_fft(int n, int count, const double* in, double* out, int inverse);

@@ -162,3 +162,39 @@ extern "C" int nlm_greens_compose(int M, int K, int Nc, const double* C2, const 
   if (err != hipSuccess) return set_err(NLM_DEVICE_ERROR, std::string("nlm_greens_compose: ") + hipGetErrorString(err));
   return NLM_OK;
 }
+
+// Cascade::computeGreensFunction (src/Cascade.cpp:110-135) on device-resident
+// Green's functions: G = G_{n-1} o ... o G_1 o G_0 with the plans' last
+// nlm_greens results (kept on the device), composed on the matrix cores without
+// host round trips; only the final C, S are copied out.
+extern "C" int nlm_greens_cascade(int n, nlm_plan* const* plans, double* C, double* S) {
+  if (n < 1 || !plans || !C || !S) return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
+  for (int i = 0; i < n; i++) {
+    if (!plans[i]) return set_err(NLM_INVALID_ARGUMENT, "null plan");
+    if (!plans[i]->greensRows) return set_err(NLM_RUNTIME_ERROR, "no Green's function has been computed");
+    HIP_TRY(hipStreamSynchronize(plans[i]->stream));
+  }
+  hipStream_t s = plans[n - 1]->stream;
+  int rows = plans[0]->greensRows;
+  const int cols = plans[0]->greensCols;
+  const cplx* aC = plans[0]->C.p;
+  const cplx* aS = plans[0]->S.p;
+  DevBuf<cplx> buf[2][2];
+  for (int i = 1; i < n; i++) {
+    const int M = plans[i]->greensRows, K = plans[i]->greensCols;
+    if (K != rows) return set_err(NLM_INVALID_ARGUMENT, "Green's function shapes of adjacent media do not match");
+    DevBuf<cplx>* o = buf[i % 2];
+    int e;
+    if ((e = o[0].ensure((size_t)M * cols)) || (e = o[1].ensure((size_t)M * cols))) return e;
+    launch_compose(plans[i]->C.p, plans[i]->S.p, aC, aS, o[0].p, o[1].p, M, K, cols, s);
+    HIP_TRY(hipGetLastError());
+    aC = o[0].p;
+    aS = o[1].p;
+    rows = M;
+  }
+  const size_t total = (size_t)rows * cols;
+  HIP_TRY(hipMemcpyAsync(C, aC, total * sizeof(cplx), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(S, aS, total * sizeof(cplx), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return NLM_OK;
+}

@@ -649,6 +649,8 @@ std::pair<Grid, Grid> Medium::greensImpl(bool inTimeDomain, bool runPump, unsign
         if (im != om) scale[(size_t)im * nOut + om] = std::sqrt(nlStep_[im] / nlStep_[om]);
   }
   std::pair<Grid, Grid> res;
+  lastGreensRows_ = nOut * N_;
+  lastGreensCols_ = nIn * N_;
   if (download) {
     res.first.setZero(nOut * N_, nIn * N_);
     res.second.setZero(nOut * N_, nIn * N_);
@@ -1263,25 +1265,27 @@ std::pair<Grid, Grid> Cascade::computeGreensFunction(bool inTimeDomain, bool run
                                                      const std::vector<uint8_t>& useInput,
                                                      const std::vector<uint8_t>& useOutput) {
   if (runPump) runPumpSimulation();
-  auto CS = media_[0]->computeGreensFunction(inTimeDomain, false, nThreads, normalize, useInput,
-                                              media_.size() == 1 ? useOutput : std::vector<uint8_t>{});
-  for (size_t i = 1; i < media_.size(); i++) {
-    auto CS2 = media_[i]->computeGreensFunction(inTimeDomain, false, nThreads, normalize, std::vector<uint8_t>{},
-                                                 i == media_.size() - 1 ? useOutput : std::vector<uint8_t>{});
-    const int M = CS2.first.rows, K = CS2.first.cols, Nc = CS.first.cols;
-    if (K != CS.first.rows) throw std::invalid_argument("Green's function shapes of adjacent media do not match");
-    std::pair<Grid, Grid> out;
-    out.first.setZero(M, Nc);
-    out.second.setZero(M, Nc);
-    check(nlm_greens_compose(M, K, Nc, reinterpret_cast<const double*>(CS2.first.v.data()),
-                             reinterpret_cast<const double*>(CS2.second.v.data()),
-                             reinterpret_cast<const double*>(CS.first.v.data()),
-                             reinterpret_cast<const double*>(CS.second.v.data()),
-                             reinterpret_cast<double*>(out.first.v.data()), reinterpret_cast<double*>(out.second.v.data())));
-    CS.first = std::move(out.first);
-    CS.second = std::move(out.second);
+  if (media_.size() == 1)
+    return media_[0]->computeGreensFunction(inTimeDomain, false, nThreads, normalize, useInput, useOutput);
+  // every medium's Green's function stays on the device; the composition
+  // (src/Cascade.cpp:128-129) runs there and only the result is copied out
+  std::vector<nlm_plan*> plans;
+  int rows = 0, cols = 0;
+  for (size_t i = 0; i < media_.size(); i++) {
+    const bool first = i == 0, last = i == media_.size() - 1;
+    Medium& m = *media_[i];
+    m.greensImpl(inTimeDomain, false, nThreads, normalize, first ? useInput : std::vector<uint8_t>{},
+                 last ? useOutput : std::vector<uint8_t>{}, false);
+    plans.push_back(m.plan());
+    if (first) cols = m.lastGreensCols_;
+    if (last) rows = m.lastGreensRows_;
   }
-  return CS;
+  std::pair<Grid, Grid> res;
+  res.first.setZero(rows, cols);
+  res.second.setZero(rows, cols);
+  check(nlm_greens_cascade((int)plans.size(), plans.data(), reinterpret_cast<double*>(res.first.v.data()),
+                           reinterpret_cast<double*>(res.second.v.data())));
+  return res;
 }
 
 Grid Cascade::batchSignalSimulation(const Grid& inputProfs, bool inTimeDomain, bool runPump, unsigned nThreads,

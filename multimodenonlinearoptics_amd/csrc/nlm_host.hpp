@@ -183,6 +183,7 @@ class Medium {
   bool signalHostStale_ = false;
   bool pumpComputed_ = false;
   int batchRows_ = 0;  // rows of the last batchForward
+  int lastGreensRows_ = 0, lastGreensCols_ = 0;  // shape of the last Green's function
 };
 
 class FullyNonlinearMedium : public Medium {

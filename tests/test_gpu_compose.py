@@ -70,3 +70,20 @@ def test_cascade_greens_matches_oracle(nlm, oracle):
     Co, So = ref.computeGreensFunction()
     assert rel_l2(Cd, Co) < 1e-10
     assert rel_l2(Sd, So) < 1e-10
+
+
+def test_three_media_cascade_matches_oracle(nlm, oracle):
+    N = 128
+    kws = [dict(relativeLength=1.0, nlLength=0.8, beta2=0.3, beta2s=-0.4, beta1=0.2, beta1s=-0.5, diffBeta0=0.7,
+                tMax=10, tPrecision=N, nZSteps=40),
+           dict(relativeLength=0.5, nlLength=1.0, beta2=-0.3, beta2s=0.2, diffBeta0=0.3, tMax=10, tPrecision=N,
+                nZSteps=33),
+           dict(relativeLength=0.7, nlLength=1.0, beta2=-0.2, beta2s=0.5, beta1=-0.3, beta1s=0.4, diffBeta0=-1.2,
+                tMax=10, tPrecision=N, nZSteps=47)]
+    dev = nlm.Cascade([nlm.Chi2PDC(**k) for k in kws], [{0: 0}, {0: 0}], False)
+    ref = oracle.Cascade([oracle.Chi2PDC(**k) for k in kws], [{0: 0}, {0: 0}], False)
+    for in_time in (False, True):
+        Cd, Sd = dev.computeGreensFunction(inTimeDomain=in_time)
+        Co, So = ref.computeGreensFunction(inTimeDomain=in_time)
+        assert rel_l2(Cd, Co) < 1e-10
+        assert rel_l2(Sd, So) < 1e-10
