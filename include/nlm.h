@@ -112,6 +112,17 @@ int nlm_pump_chi3(nlm_plan* plan, const double* envelope, const double* disp_pum
 int nlm_pump_upload(nlm_plan* plan, int pump_index, const double* pump_time, const double* pump_freq);
 int nlm_pump_download(nlm_plan* plan, int pump_index, double* pump_time, double* pump_freq);
 
+/* _NonlinearMedium::setPump(other, modeIndex, delay) (src/_NonlinearMedium.cpp:506-534)
+ * on the device: pump grid `pump_index` of plan built from the signal grid of
+ * mode `mode` of src (its last nlm_propagate_grid), without host copies.
+ * disp_pump: [N] real D_p of this medium's pump; disp_signal_src: [N] real D_s of
+ * the source mode; omega: [N]; beta1_delay = beta1 of the pump * delay; z, z_src
+ * the two media's lengths.  As in the reference, the last pump time row keeps its
+ * previous contents. */
+int nlm_pump_import(nlm_plan* plan, int pump_index, nlm_plan* src, int mode, const double* disp_pump,
+                    const double* disp_signal_src, const double* omega, double dzp, double dz_src,
+                    double beta1_delay, double z, double z_src);
+
 /* Full z-resolved propagation, runSignalSimulation -> signalSimulationTemplate
  * with optimized=false (src/_NonlinearMedium.cpp:253-260,
  * src/_NonlinearMedium.hpp:219-238).  input: in_len complex (N or k*N). */

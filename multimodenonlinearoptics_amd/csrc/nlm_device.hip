@@ -95,6 +95,61 @@ __global__ void __launch_bounds__(Launch<L>::THREADS)
   store_row<L>(pumpT + (size_t)i * N, tm, t, active);
 }
 
+// Pump rows from another medium's signal grid, setPump(other, modeIndex, delay)
+// (src/_NonlinearMedium.cpp:506-534): row i < nZP-1 interpolates the source's
+// frequency rows j, j+1 at j + frac = i (nzs-1)/(nZP-1) with the frame phase
+// exp(i ((i dzp) D_p - ((j + 1/2) dzs) D_s + bd omega)) and is transformed back
+// to time; the last row takes the source's last row with exp(i (z D_p - zs D_s +
+// bd omega)) and, as in the reference, its time row is left as it was.
+template <int L>
+__global__ void __launch_bounds__(Launch<L>::THREADS)
+    k_pump_import(const cplx* __restrict__ sigF, int nzs, double dzs, const double* __restrict__ Ds,
+                  const double* __restrict__ Dp, const double* __restrict__ omega, double dzp, double bd, double z,
+                  double zs, int nZP, cplx* __restrict__ pumpF, cplx* __restrict__ pumpT,
+                  const cplx* __restrict__ twt) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T, FPB = Launch<L>::FPB;
+  __shared__ cplx lds[FPB * F::lds_elems];
+  const int lf = threadIdx.x / T, t = threadIdx.x % T;
+  const int row = blockIdx.x * FPB + lf;
+  const bool active = row < nZP;
+  cplx* buf = lds + lf * F::lds_elems;
+  FftTwiddles<L> tw;
+  fft_load_twiddles<L>(tw, twt, t);
+  const int i = active ? row : 0;
+  const bool lastRow = i == nZP - 1;
+  cplx f[8];
+  if (!lastRow) {
+    const double j_ = i * ((double)(nzs - 1) / (nZP - 1));
+    const int j = (int)j_;
+    const double frac = j_ - j;
+    const cplx* r0 = sigF + (size_t)j * N;
+    const cplx* r1 = r0 + N;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      const int k = t + T * m;
+      const double ph = ((double)i * dzp) * Dp[k] - ((j_ + 0.5) * dzs) * Ds[k] + bd * omega[k];
+      double sn, cs;
+      sincos_reduced(ph, &sn, &cs);
+      f[m] = ((1 - frac) * r0[k] + frac * r1[k]) * mk(cs, sn);
+    }
+  } else {
+    const cplx* r = sigF + (size_t)(nzs - 1) * N;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      const int k = t + T * m;
+      const double ph = z * Dp[k] - zs * Ds[k] + bd * omega[k];
+      double sn, cs;
+      sincos_reduced(ph, &sn, &cs);
+      f[m] = r[k] * mk(cs, sn);
+    }
+  }
+  store_row<L>(pumpF + (size_t)i * N, f, t, active);
+  fft_field<L, true>(f, buf, t, tw);
+  scale_inv<L>(f);
+  store_row<L>(pumpT + (size_t)i * N, f, t, active && !lastRow);
+}
+
 // Chi3 nonlinear pump, serial over 2nZ-1 rows (solvers/Chi3.cpp:25-54).
 template <int L>
 __global__ void __launch_bounds__(FftShape<L>::T) k_pump_chi3(const cplx* __restrict__ env, const double* __restrict__ Dp,
@@ -554,6 +609,45 @@ extern "C" int nlm_pump_upload(nlm_plan* p, int q, const double* pump_time, cons
   if (pump_time) HIP_TRY(hipMemcpyAsync(p->pumpT[q].p, pump_time, cnt * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
   if (pump_freq) HIP_TRY(hipMemcpyAsync(p->pumpF[q].p, pump_freq, cnt * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
+  p->pumpLinear = false;
+  return NLM_OK;
+}
+
+typedef void (*import_fn)(const cplx*, int, double, const double*, const double*, const double*, double, double,
+                          double, double, int, cplx*, cplx*, const cplx*, hipStream_t);
+template <int L>
+static void launch_import(const cplx* sigF, int nzs, double dzs, const double* Ds, const double* Dp,
+                          const double* omega, double dzp, double bd, double z, double zs, int nZP, cplx* pF, cplx* pT,
+                          const cplx* tw, hipStream_t s) {
+  const int grid = (nZP + Launch<L>::FPB - 1) / Launch<L>::FPB;
+  hipLaunchKernelGGL((k_pump_import<L>), dim3(grid), dim3(Launch<L>::THREADS), 0, s, sigF, nzs, dzs, Ds, Dp, omega,
+                     dzp, bd, z, zs, nZP, pF, pT, tw);
+}
+static import_fn pick_import(int L) { NLM_L_SWITCH(launch_import, L) }
+
+extern "C" int nlm_pump_import(nlm_plan* p, int q, nlm_plan* src, int mode, const double* disp_pump,
+                               const double* disp_signal_src, const double* omega, double dzp, double dz_src,
+                               double beta1_delay, double z, double z_src) {
+  if (!p || !src || !disp_pump || !disp_signal_src || !omega) return set_err(NLM_INVALID_ARGUMENT, "null argument");
+  if (q < 0 || q >= p->nP) return set_err(NLM_INVALID_ARGUMENT, "Invalid pump index");
+  if (src->N != p->N) return set_err(NLM_INVALID_ARGUMENT, "Medium does not have same time and frequency axes as this one");
+  if (mode < 0 || mode >= src->nS) return set_err(NLM_INVALID_ARGUMENT, "Mode index larger than number of modes in medium");
+  if (src->nZ < p->nZ) return set_err(NLM_INVALID_ARGUMENT, "Medium does not have sufficient resolution to be used with this one");
+  if (!src->gridF.p) return set_err(NLM_RUNTIME_ERROR, "no signal grid has been computed");
+  const int N = p->N, nZP = 2 * p->nZ - 1;
+  if (int e = ensure_pumps(p)) return e;
+  DevBuf<double> tab;
+  if (int e = tab.ensure(3 * (size_t)N)) return e;
+  HIP_TRY(hipMemcpyAsync(tab.p, disp_pump, N * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  HIP_TRY(hipMemcpyAsync(tab.p + N, disp_signal_src, N * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  HIP_TRY(hipMemcpyAsync(tab.p + 2 * N, omega, N * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  HIP_TRY(hipStreamSynchronize(src->stream));  // the source grid is complete
+  const import_fn launch = pick_import(p->L);
+  if (!launch) return set_err(NLM_INVALID_ARGUMENT, "unsupported size for the device pump import");
+  launch(src->gridF.p + (size_t)mode * src->nZ * N, src->nZ, dz_src, tab.p + N, tab.p, tab.p + 2 * N, dzp,
+         beta1_delay, z, z_src, nZP, p->pumpF[q].p, p->pumpT[q].p, p->tw, p->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(p->stream));  // before the temporary tables are freed
   p->pumpLinear = false;
   return NLM_OK;
 }

@@ -294,11 +294,22 @@ void Medium::setPump(const Medium& otherC, unsigned modeIndex, double delayLengt
   if ((int)modeIndex >= nS_) throw std::invalid_argument("Mode index larger than number of modes in medium");
   if ((int)pumpIndex >= nP_) throw std::invalid_argument("Invalid pump index");
   if (other.nZ_ < nZ_) throw std::invalid_argument("Medium does not have sufficient resolution to be used with this one");
+  const double bd = beta1_[pumpIndex] * delayLength;
+  if (other.signalOnDevice_ && other.plan_) {
+    // the source grid is on the device: build the pump grid there
+    // (the last pump time row keeps the device copy of its current contents)
+    if (pumpDevStale_[pumpIndex] || !pumpComputed_) uploadPump(pumpIndex);
+    check(nlm_pump_import(plan(), (int)pumpIndex, other.plan_, (int)modeIndex, dispPump_[pumpIndex].data(),
+                          other.dispSign_[modeIndex].data(), omega_.data(), dzp_, other.dz_, bd, z_, other.z_));
+    pumpHostStale_[pumpIndex] = true;
+    pumpDevStale_[pumpIndex] = false;
+    pumpComputed_ = true;
+    return;
+  }
   syncPumpToHost(pumpIndex);
   const Grid& oF = other.signalFreq(modeIndex);
   Grid& pF = pumpFreq_[pumpIndex];
   Grid& pT = pumpTime_[pumpIndex];
-  const double bd = beta1_[pumpIndex] * delayLength;
   for (unsigned i = 0; i < (unsigned)nZP_ - 1; i++) {
     const double j_ = i * (static_cast<double>(other.nZ_ - 1) / (nZP_ - 1));
     const unsigned j = static_cast<unsigned>(j_);
@@ -504,6 +515,7 @@ void Medium::runSignalSimulation(const std::vector<cd>& in, bool inTimeDomain, u
   check(nlm_propagate_grid(p, reinterpret_cast<const double*>(in.data()), (int)n, inTimeDomain ? 1 : 0, (int)inputMode));
   check(nlm_plan_sync(p));
   signalHostStale_ = true;
+  signalOnDevice_ = true;
 }
 
 // Mode selection of computeGreensFunction, src/_NonlinearMedium.cpp:266-299.

@@ -181,6 +181,7 @@ class Medium {
   bool coefDirty_ = false;
   std::vector<bool> pumpHostStale_, pumpDevStale_;
   bool signalHostStale_ = false;
+  bool signalOnDevice_ = false;  // the plan holds the signal grids of the last runSignalSimulation
   bool pumpComputed_ = false;
   int batchRows_ = 0;  // rows of the last batchForward
   int lastGreensRows_ = 0, lastGreensCols_ = 0;  // shape of the last Green's function
