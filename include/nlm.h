@@ -127,6 +127,12 @@ int nlm_pump_import(nlm_plan* plan, int pump_index, nlm_plan* src, int mode, con
  * with optimized=false (src/_NonlinearMedium.cpp:253-260,
  * src/_NonlinearMedium.hpp:219-238).  input: in_len complex (N or k*N). */
 int nlm_propagate_grid(nlm_plan* plan, const double* input, int in_len, int in_time_domain, int input_mode);
+/* z-resolved batches (extension): the full-grid propagation of every one of rows
+ * input rows (rows x cols, as nlm_batch) in one launch; grid_freq, grid_time:
+ * [rows][nS][nZ][N] complex (host), or NULL to keep them on the device.  Row 0's
+ * grids are then also what nlm_grid_download returns. */
+int nlm_batch_grids(nlm_plan* plan, const double* inputs, int rows, int cols, int in_time_domain, int input_mode,
+                    double* grid_freq, double* grid_time);
 /* [nZ][N] complex signal grids of one mode from the last nlm_propagate_grid. */
 int nlm_grid_download(nlm_plan* plan, int mode, double* signal_freq, double* signal_time);
 

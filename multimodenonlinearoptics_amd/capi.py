@@ -19,6 +19,7 @@ SYMBOLS = [
     "nlm_multi_create", "nlm_multi_launch", "nlm_multi_sync", "nlm_multi_output", "nlm_multi_classes",
     "nlm_multi_grid", "nlm_multi_destroy", "nlm_batch_forward", "nlm_batch_adjoint",
     "nlm_greens_compose_dev", "nlm_greens_cascade", "nlm_pump_import",
+    "nlm_batch_grids",
 ]
 
 _lib = None

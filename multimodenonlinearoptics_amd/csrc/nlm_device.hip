@@ -757,6 +757,43 @@ extern "C" int nlm_propagate_grid(nlm_plan* p, const double* input, int in_len, 
   return NLM_OK;
 }
 
+// z-resolved batches: runSignalSimulation (src/_NonlinearMedium.cpp:253-260,
+// src/_NonlinearMedium.hpp:219-238) of every input row in one launch.
+extern "C" int nlm_batch_grids(nlm_plan* p, const double* inputs, int rows, int cols, int in_time, int input_mode,
+                               double* grid_freq, double* grid_time) {
+  if (!p || (rows > 0 && !inputs)) return set_err(NLM_INVALID_ARGUMENT, "null argument");
+  if (rows < 0) return set_err(NLM_INVALID_ARGUMENT, "negative row count");
+  const int N = p->N;
+  if (cols % N != 0 || cols / N == 0 || cols / N > p->nS)
+    return set_err(NLM_INVALID_ARGUMENT, "inputProf array size does not match number of frequency/time bins");
+  if (input_mode < 0 || input_mode >= p->nS)
+    return set_err(NLM_INVALID_ARGUMENT, "inputModes does not match any mode in the system");
+  if (rows == 0) return NLM_OK;
+  if (int e = check_pumps_ready(p)) return e;
+  const size_t g = (size_t)rows * p->nS * p->nZ * N;
+  if (int e = p->gridF.ensure(g)) return e;
+  if (int e = p->gridT.ensure(g)) return e;
+  if (int e = upload(p->in, inputs, (size_t)rows * cols, p->stream)) return e;
+  KParams kp;
+  fill_common(p, kp);
+  kp.kind = KIND_GRID;
+  kp.nFields = rows;
+  kp.inTime = in_time;
+  kp.inCols = cols;
+  kp.in = p->in.p;
+  kp.gridF = p->gridF.p;
+  kp.gridT = p->gridT.p;
+  input_channels(p->nS, cols / N, input_mode, kp.inPos);
+  tic(p, 2);
+  if (int e = launch_prop_kernel(p, kp)) return e;
+  tic(p, 3);
+  if (grid_freq) HIP_TRY(hipMemcpyAsync(grid_freq, p->gridF.p, g * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  if (grid_time) HIP_TRY(hipMemcpyAsync(grid_time, p->gridT.p, g * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  if (p->timing) collect_timing(p, 2, 3, 1);
+  return NLM_OK;
+}
+
 extern "C" int nlm_grid_download(nlm_plan* p, int mode, double* sf, double* st) {
   if (!p) return set_err(NLM_INVALID_ARGUMENT, "null plan");
   if (mode < 0 || mode >= p->nS) return set_err(NLM_INVALID_ARGUMENT, "mode index out of range");

@@ -518,6 +518,29 @@ void Medium::runSignalSimulation(const std::vector<cd>& in, bool inTimeDomain, u
   signalOnDevice_ = true;
 }
 
+// runSignalSimulation (src/_NonlinearMedium.cpp:253-260) of every input row at once.
+std::pair<Grid, Grid> Medium::batchSignalGrids(const Grid& in, bool inTimeDomain, unsigned inputMode) {
+  if (in.cols % N_ != 0 || in.cols / N_ == 0 || in.cols / N_ > nS_)
+    throw std::invalid_argument("inputProf array size does not match number of frequency/time bins");
+  if ((int)inputMode >= nS_) throw std::invalid_argument("inputModes does not match any mode in the system");
+  nlm_plan* p = plan();
+  ensure_pumps_on_device(this, pumpDevStale_, pumpComputed_, nP_, &Medium::uploadPump);
+  if (nP_ > 0 && !pumpComputed_) {
+    for (int q = 0; q < nP_; q++) uploadPump(q);
+    pumpComputed_ = true;
+  }
+  std::pair<Grid, Grid> g;
+  g.first.setZero(in.rows * nS_ * nZ_, N_);
+  g.second.setZero(in.rows * nS_ * nZ_, N_);
+  check(nlm_batch_grids(p, reinterpret_cast<const double*>(in.v.data()), in.rows, in.cols, inTimeDomain ? 1 : 0,
+                        (int)inputMode, reinterpret_cast<double*>(g.first.v.data()),
+                        reinterpret_cast<double*>(g.second.v.data())));
+  // the plan's grid now holds row 0's propagation
+  signalHostStale_ = true;
+  signalOnDevice_ = in.rows > 0;
+  return g;
+}
+
 // Mode selection of computeGreensFunction, src/_NonlinearMedium.cpp:266-299.
 // The reference counts entries below nSignalModes but selects by flag; the two
 // agree only for flag lists whose set entries are all < nSignalModes, which is

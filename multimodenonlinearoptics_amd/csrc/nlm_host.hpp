@@ -104,6 +104,9 @@ class Medium {
   std::pair<std::vector<cd>, std::vector<double>> greensAdjoint(const Grid& gC, const Grid& gS);
   // differentiable batch (Chi3GNLSE): forward keeping checkpoints, then dL/d(inputs)
   Grid batchForward(const Grid& inputProfs, bool inTimeDomain, int ckptEvery);
+  // z-resolved batch: runSignalSimulation of every row in one launch; grids of
+  // shape (rows * nS * nZ, N), frequency and time domain
+  std::pair<Grid, Grid> batchSignalGrids(const Grid& inputProfs, bool inTimeDomain, unsigned inputMode);
   Grid batchAdjoint(const Grid& gOut);
   void enableTiming(bool on);
   void setStepTable(bool on);

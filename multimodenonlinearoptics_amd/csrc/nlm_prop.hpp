@@ -35,7 +35,7 @@ struct KParams {
   int nOut;
   const cplx* in;
   cplx* out;           // [nFields][nOut*N]
-  cplx* gridF;         // [nS][nZ][N]
+  cplx* gridF;         // [nFields][nS][nZ][N] (z-resolved runs)
   cplx* gridT;
   const cplx* tw;
   const cplx* dstep;   // [nS][N]
@@ -65,6 +65,11 @@ struct Launch {
   static constexpr int FPB = T >= 64 ? 1 : 64 / T;  // fields per workgroup
   static constexpr int THREADS = FPB * T;
 };
+
+// Row i of mode q of the z-resolved grid of `field` ([nFields][nS][nZ][N]).
+__device__ __forceinline__ cplx* grid_row(const KParams& kp, cplx* base, int field, int nS, int q, int i, int N) {
+  return base + (((size_t)field * nS + q) * kp.nZ + i) * N;
+}
 
 // ---------------------------------------------------------------------------
 // field I/O helpers (stride layout: lane t holds k = t + T*m)
@@ -127,7 +132,7 @@ __device__ __forceinline__ void load_inputs(cplx (&a)[NS][8], const KParams& kp,
         for (int m = 0; m < 8; m++)
           if (t + T * m == pos) a[q][m] = val;
   } else {
-    const cplx* row = kp.in + (size_t)(kp.kind == KIND_BATCH ? field : 0) * kp.inCols;
+    const cplx* row = kp.in + (size_t)field * kp.inCols;
 #pragma unroll
     for (int q = 0; q < NS; q++) {
       const int ch = kp.inPos[q];
@@ -142,13 +147,13 @@ __device__ __forceinline__ void load_inputs(cplx (&a)[NS][8], const KParams& kp,
 // the grid path records row 0 of the frequency grid.
 template <int L, int NS>
 __device__ __forceinline__ void prologue(cplx (&a)[NS][8], const KParams& kp, cplx* buf, int t,
-                                         const FftTwiddles<L>& tw, bool active) {
+                                         const FftTwiddles<L>& tw, int field, bool active) {
   constexpr int N = FftShape<L>::N;
 #pragma unroll
   for (int q = 0; q < NS; q++) {
     if (kp.inTime) fft_field<L, false>(a[q], buf, t, tw);
     mul_table<L>(a[q], kp.halfIn + (size_t)q * N, t);
-    if (kp.kind == KIND_GRID) store_row<L>(kp.gridF + (size_t)q * kp.nZ * N, a[q], t, active);
+    if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, NS, q, 0, N), a[q], t, active);
   }
 }
 
@@ -163,10 +168,10 @@ __device__ __forceinline__ void epilogue(cplx (&a)[NS][8], const KParams& kp, cp
   for (int q = 0; q < NS; q++) {
     mul_table<L>(a[q], kp.halfOut + (size_t)q * N, t);
     if (kp.kind == KIND_GRID) {
-      store_row<L>(kp.gridF + ((size_t)q * kp.nZ + last) * N, a[q], t, active);
+      store_row<L>(grid_row(kp, kp.gridF, field, NS, q, last, N), a[q], t, active);
       fft_field<L, true>(a[q], buf, t, tw);
       scale_inv<L>(a[q]);
-      store_row<L>(kp.gridT + ((size_t)q * kp.nZ + last) * N, a[q], t, active);
+      store_row<L>(grid_row(kp, kp.gridT, field, NS, q, last, N), a[q], t, active);
     } else {
       const int op = kp.outPos[q];
       if (kp.outTime) {
@@ -249,13 +254,13 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
 
   cplx a[NS][8];
   load_inputs<L, NS>(a, kp, field, t, active);
-  prologue<L, NS>(a, kp, buf, t, tw, active);
+  prologue<L, NS>(a, kp, buf, t, tw, field, active);
   if (kp.nZ > 1) {
 #pragma unroll
     for (int q = 0; q < NS; q++) {
       fft_field<L, true>(a[q], buf, t, tw);
       scale_inv<L>(a[q]);
-      if (kp.kind == KIND_GRID) store_row<L>(kp.gridT + (size_t)q * kp.nZ * N, a[q], t, active);
+      if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, NS, q, 0, N), a[q], t, active);
       if (kp.ckpt) store_row<L>(kp.ckpt + ((size_t)field * kp.nSeg * NS + q) * N, a[q], t, active);
     }
   }
@@ -332,10 +337,10 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
 #pragma unroll
       for (int m = 0; m < 8; m++) a[q][m] = a[q][m] * ds[q][m];
       if (i < kp.nZ - 1) {
-        if (kp.kind == KIND_GRID) store_row<L>(kp.gridF + ((size_t)q * kp.nZ + i) * N, a[q], t, active);
+        if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, NS, q, i, N), a[q], t, active);
         fft_field<L, true>(a[q], buf, t, tw);
         scale_inv<L>(a[q]);
-        if (kp.kind == KIND_GRID) store_row<L>(kp.gridT + ((size_t)q * kp.nZ + i) * N, a[q], t, active);
+        if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, NS, q, i, N), a[q], t, active);
         if (kp.ckpt && i % kp.ckptEvery == 0)
           store_row<L>(kp.ckpt + (((size_t)field * kp.nSeg + i / kp.ckptEvery) * NS + q) * N, a[q], t, active);
       }
@@ -402,11 +407,11 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_gnlse(KParams kp) {
 
   cplx a[1][8];
   load_inputs<L, 1>(a, kp, field, t, active);
-  prologue<L, 1>(a, kp, buf, t, tw, active);
+  prologue<L, 1>(a, kp, buf, t, tw, field, active);
   if (kp.nZ > 1) {
     fft_field<L, true>(a[0], buf, t, tw);
     scale_inv<L>(a[0]);
-    if (kp.kind == KIND_GRID) store_row<L>(kp.gridT, a[0], t, active);
+    if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, 1, 0, 0, N), a[0], t, active);
   }
 
   for (int i = 1; i < kp.nZ; i++) {
@@ -440,10 +445,10 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_gnlse(KParams kp) {
 #pragma unroll
     for (int m = 0; m < 8; m++) a[0][m] = a[0][m] * ds[m];
     if (i < kp.nZ - 1) {
-      if (kp.kind == KIND_GRID) store_row<L>(kp.gridF + (size_t)i * N, a[0], t, active);
+      if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, 1, 0, i, N), a[0], t, active);
       fft_field<L, true>(a[0], buf, t, tw);
       scale_inv<L>(a[0]);
-      if (kp.kind == KIND_GRID) store_row<L>(kp.gridT + (size_t)i * N, a[0], t, active);
+      if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, 1, 0, i, N), a[0], t, active);
     }
   }
   epilogue<L, 1>(a, kp, buf, t, tw, field, active);

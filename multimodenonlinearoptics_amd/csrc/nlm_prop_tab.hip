@@ -92,11 +92,11 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
 
   cplx a[1][8];
   load_inputs<L, 1>(a, kp, field, t, active);
-  prologue<L, 1>(a, kp, buf, t, tw, active);
+  prologue<L, 1>(a, kp, buf, t, tw, field, active);
   cplx al[8], be[8];
   if (kp.nZ > 1) {
     fft_field<L, true>(a[0], buf, t, tw);  // unscaled: the table holds the 1/N
-    if (kp.kind == KIND_GRID) store_scaled<L>(kp.gridT, a[0], t, active);
+    if (kp.kind == KIND_GRID) store_scaled<L>(grid_row(kp, kp.gridT, field, 1, 0, 0, N), a[0], t, active);
     if (kp.ckpt) store_scaled<L>(kp.ckpt + (size_t)field * kp.nSeg * N, a[0], t, active);
     load_prop_row<L>(al, be, kp.tab + (size_t)N * 2, t);
   }
@@ -116,9 +116,9 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
 #pragma unroll
     for (int m = 0; m < 8; m++) a[0][m] = a[0][m] * ds[m];
     if (i < kp.nZ - 1) {
-      if (kp.kind == KIND_GRID) store_row<L>(kp.gridF + (size_t)i * N, a[0], t, active);
+      if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, 1, 0, i, N), a[0], t, active);
       fft_field<L, true>(a[0], buf, t, tw);
-      if (kp.kind == KIND_GRID) store_scaled<L>(kp.gridT + (size_t)i * N, a[0], t, active);
+      if (kp.kind == KIND_GRID) store_scaled<L>(grid_row(kp, kp.gridT, field, 1, 0, i, N), a[0], t, active);
       if (kp.ckpt && i % kp.ckptEvery == 0)
         store_scaled<L>(kp.ckpt + ((size_t)field * kp.nSeg + i / kp.ckptEvery) * N, a[0], t, active);
     }

@@ -259,6 +259,22 @@ PYBIND11_MODULE(nonlinearmedium, m) {
            },
            "Gradients (dL/d envelope, dL/d poling profile) of the last greensForward for dL/dC, dL/dS "
            "(convention dL/dRe + i dL/dIm) (extension; Chi2PDC).", "gC"_a, "gS"_a);
+  base.def("batchSignalGrids",
+           [](Medium& md, const carr& in, bool inTimeDomain, unsigned inputMode) {
+             Grid g = to_grid(in);
+             std::pair<Grid, Grid> r;
+             {
+               py::gil_scoped_release nogil;
+               r = md.batchSignalGrids(g, inTimeDomain, inputMode);
+             }
+             const py::ssize_t rows = g.rows, nS = (py::ssize_t)md.nSignalModes(), nZ = (py::ssize_t)md.nZSteps(),
+                               N = (py::ssize_t)md.nFreqs();
+             py::array f = owned(std::move(r.first)), t = owned(std::move(r.second));
+             return py::make_tuple(f.reshape({rows, nS, nZ, N}), t.reshape({rows, nS, nZ, N}));
+           },
+           "runSignalSimulation of every input row in one launch (extension): signal grids "
+           "[rows, nSignalModes, nZSteps, nFreqs] in the frequency and time domains.",
+           "inputProfs"_a, "inTimeDomain"_a = true, "inputMode"_a = 0);
   base.def("batchForward",
            [](Medium& md, const carr& in, bool inTimeDomain, int ckptEvery) {
              Grid g = to_grid(in);
