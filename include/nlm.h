@@ -250,6 +250,15 @@ int nlm_greens_compose_dev(int M, int K, int Ncol, const double* C2, const doubl
  * result (rows of plans[n-1]) x (cols of plans[0]) written to C, S (host). */
 int nlm_greens_cascade(int n, nlm_plan* const* plans, double* C, double* S);
 
+/* Gaussian-state quantities of a Green's function (extension; the reference's
+ * multimode.calcQuadratureGreens / calcCovarianceMtx, multimode.py:15-36, hbar = 2):
+ * Z = [[Re(C+S), -Im(C-S)], [Im(C+S), Re(C-S)]] (2N x 2N real) and cov = Z Z^T.
+ * C, S: N x N complex (host); Z, cov: 2N x 2N doubles (host), either may be NULL. */
+int nlm_gaussian_covariance(int N, const double* C, const double* S, double* Z, double* cov);
+/* out = A A^T for a real row-major M x K matrix A (host buffers), e.g. the
+ * covariance of a given quadrature matrix (multimode.calcCovarianceMtx). */
+int nlm_gram(int M, int K, const double* A, double* out);
+
 /* Standalone transform with the reference's conventions (src/CustomEigenFFT.h):
  * count rows of length n, forward unscaled / inverse scaled by 1/n. */
 int nlm_fft(int n, int count, const double* in, double* out, int inverse);

@@ -19,7 +19,7 @@ SYMBOLS = [
     "nlm_multi_create", "nlm_multi_launch", "nlm_multi_sync", "nlm_multi_output", "nlm_multi_classes",
     "nlm_multi_grid", "nlm_multi_destroy", "nlm_batch_forward", "nlm_batch_adjoint",
     "nlm_greens_compose_dev", "nlm_greens_cascade", "nlm_pump_import",
-    "nlm_batch_grids",
+    "nlm_batch_grids", "nlm_gaussian_covariance", "nlm_gram",
 ]
 
 _lib = None
@@ -58,6 +58,8 @@ def lib():
         L.nlm_batch.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, vp]
         L.nlm_batch_dev.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, vp]
         L.nlm_greens_compose.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.nlm_gaussian_covariance.argtypes = [C.c_int, vp, vp, vp, vp]
+        L.nlm_gram.argtypes = [C.c_int, C.c_int, vp, vp]
         L.nlm_greens_compose_dev.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib

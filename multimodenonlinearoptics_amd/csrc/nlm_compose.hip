@@ -198,3 +198,98 @@ extern "C" int nlm_greens_cascade(int n, nlm_plan* const* plans, double* C, doub
   HIP_TRY(hipStreamSynchronize(s));
   return NLM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Gaussian-state quantities of a Green's function (multimode.py:15-36 of the
+// reference, downstream of computeGreensFunction): the quadrature transfer
+// matrix Z = [[Re(C+S), -Im(C-S)], [Im(C+S), Re(C-S)]] and the covariance Z Z^T
+// (real, 2N x 2N), the product on the FP64 matrix cores.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+__global__ void k_quadrature(const cplx* __restrict__ C, const cplx* __restrict__ S, int N, double* __restrict__ Z) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)N * N) return;
+  const int r = (int)(idx / N), c = (int)(idx % N);
+  const cplx a = C[idx], b = S[idx];
+  const size_t W = 2 * (size_t)N;
+  Z[(size_t)r * W + c] = a.re + b.re;            // Re(C + S)
+  Z[(size_t)r * W + N + c] = -(a.im - b.im);      // -Im(C - S)
+  Z[(size_t)(N + r) * W + c] = a.im + b.im;      // Im(C + S)
+  Z[(size_t)(N + r) * W + N + c] = a.re - b.re;  // Re(C - S)
+}
+
+// out = A A^T for a real row-major M x K matrix A: 32 x 32 tiles, four waves of
+// 16 x 16 FP64 MFMAs (operand layout as in k_compose_mfma).
+__global__ void __launch_bounds__(256) k_gram_mfma(const double* __restrict__ A, int M, int K,
+                                                   double* __restrict__ out) {
+  __shared__ double Ar[TILE][APAD], Br[TILE][APAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int row0 = blockIdx.y * TILE, col0 = blockIdx.x * TILE;
+  const int wr = (wave >> 1) * 16, wc = (wave & 1) * 16;
+  v4d acc = {0, 0, 0, 0};
+  for (int k0 = 0; k0 < K; k0 += KB) {
+    for (int e = tid; e < TILE * KB; e += 256) {
+      const int r = e / KB, k = e % KB, gk = k0 + k;
+      Ar[r][k] = (row0 + r < M && gk < K) ? A[(size_t)(row0 + r) * K + gk] : 0.0;
+      Br[r][k] = (col0 + r < M && gk < K) ? A[(size_t)(col0 + r) * K + gk] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KB; kk += 4) {
+      const int k = kk + (lane >> 4);
+      acc = mfma(Ar[wr + (lane & 15)][k], Br[wc + (lane & 15)][k], acc);
+    }
+    __syncthreads();
+  }
+  const int oc = col0 + wc + (lane & 15);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int orow = row0 + wr + (lane >> 4) + 4 * r;
+    if (orow < M && oc < M) out[(size_t)orow * M + oc] = acc[r];
+  }
+}
+
+}  // namespace
+
+extern "C" int nlm_gaussian_covariance(int N, const double* C, const double* S, double* Z, double* cov) {
+  if (N <= 0 || !C || !S || (!Z && !cov)) return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
+  const size_t n2 = (size_t)N * N, W = 2 * (size_t)N;
+  DevBuf<cplx> dC, dS;
+  DevBuf<double> dZ, dV;
+  int e;
+  if ((e = dC.ensure(n2)) || (e = dS.ensure(n2)) || (e = dZ.ensure(W * W)) || (cov && (e = dV.ensure(W * W)))) return e;
+  hipError_t err = hipMemcpy(dC.p, C, n2 * sizeof(cplx), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(dS.p, S, n2 * sizeof(cplx), hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    hipLaunchKernelGGL(k_quadrature, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, 0, dC.p, dS.p, N, dZ.p);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess && cov) {
+    const dim3 grid((unsigned)((W + TILE - 1) / TILE), (unsigned)((W + TILE - 1) / TILE));
+    hipLaunchKernelGGL(k_gram_mfma, grid, dim3(256), 0, 0, dZ.p, (int)W, (int)W, dV.p);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess && Z) err = hipMemcpy(Z, dZ.p, W * W * sizeof(double), hipMemcpyDeviceToHost);
+  if (err == hipSuccess && cov) err = hipMemcpy(cov, dV.p, W * W * sizeof(double), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return set_err(NLM_DEVICE_ERROR, std::string("nlm_gaussian_covariance: ") + hipGetErrorString(err));
+  return NLM_OK;
+}
+
+extern "C" int nlm_gram(int M, int K, const double* A, double* out) {
+  if (M < 0 || K < 0 || (M > 0 && (!A || !out))) return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
+  if (M == 0) return NLM_OK;
+  DevBuf<double> dA, dO;
+  int e;
+  if ((e = dA.ensure((size_t)M * (K > 0 ? K : 1))) || (e = dO.ensure((size_t)M * M))) return e;
+  hipError_t err = K > 0 ? hipMemcpy(dA.p, A, (size_t)M * K * sizeof(double), hipMemcpyHostToDevice) : hipSuccess;
+  if (err == hipSuccess) {
+    const dim3 grid((unsigned)((M + TILE - 1) / TILE), (unsigned)((M + TILE - 1) / TILE));
+    hipLaunchKernelGGL(k_gram_mfma, grid, dim3(256), 0, 0, dA.p, M, K, dO.p);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipMemcpy(out, dO.p, (size_t)M * M * sizeof(double), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return set_err(NLM_DEVICE_ERROR, std::string("nlm_gram: ") + hipGetErrorString(err));
+  return NLM_OK;
+}

@@ -1,0 +1,42 @@
+"""Gaussian-state quantities of Green's functions on the device
+(multimodenonlinearoptics_amd.multimode, after the reference's multimode.py:15-36):
+the quadrature matrix is exact, the covariance Z Z^T matches numpy, and for a
+Chi2PDC Green's function (a symplectic map) the covariance has determinant 1."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _quad_numpy(C, S):
+    return np.block([[np.real(C + S), -np.imag(C - S)], [np.imag(C + S), np.real(C - S)]])
+
+
+def test_quadrature_and_gram_match_numpy(nlm):
+    from multimodenonlinearoptics_amd import capi, multimode
+    r = np.random.default_rng(21)
+    n = 77
+    C = r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))
+    S = r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))
+    Z = multimode.calcQuadratureGreens(C, S)
+    assert np.array_equal(Z, _quad_numpy(C, S))
+    for m, k in [(1, 1), (33, 70), (154, 154), (200, 31)]:
+        A = r.standard_normal((m, k))
+        out = np.empty((m, m))
+        capi.check(capi.lib().nlm_gram(m, k, A.ctypes.data, out.ctypes.data))
+        assert rel_l2(out, A @ A.T) < 1e-13
+
+
+def test_covariance_of_pdc_greens_is_symplectic(nlm):
+    from multimodenonlinearoptics_amd import multimode
+    m = nlm.Chi2PDC(relativeLength=1.0, nlLength=1.0, beta2=0.3, beta2s=-0.2, diffBeta0=0.5, tMax=10,
+                    tPrecision=128, nZSteps=400)
+    C, S = m.computeGreensFunction()
+    Z = multimode.calcQuadratureGreens(C, S)
+    cov = multimode.calcCovarianceMtx(Z, tol=1e-6)
+    assert rel_l2(cov, Z @ Z.T) < 1e-13
+    assert abs(np.linalg.det(cov) - 1) < 1e-6
+    with pytest.raises(AssertionError):
+        multimode.calcCovarianceMtx(2 * Z)
