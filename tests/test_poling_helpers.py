@@ -1,0 +1,48 @@
+"""Poling design helpers (multimodenonlinearoptics_amd.poling) against the
+oracle's setPoling discretisation (src/_NonlinearMedium.cpp:470-503) and
+round trips; CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from multimodenonlinearoptics_amd import poling as P
+
+
+def _oracle_profile(oracle, domains, nZ):
+    m = oracle.Chi2PDC(relativeLength=1.0, nlLength=1.0, beta2=0.3, beta2s=-0.2, tMax=10, tPrecision=16,
+                       nZSteps=nZ, poling=list(domains))
+    return np.asarray(m.poling)
+
+
+@pytest.mark.parametrize("seed,n,nZ", [(1, 7, 100), (2, 30, 257), (3, 2, 10), (4, 1, 12)])
+def test_domains_to_space_matches_set_poling(oracle, seed, n, nZ):
+    d = np.random.default_rng(seed).uniform(0.5, 2.0, size=n)
+    np.testing.assert_array_equal(P.domainsToSpace(d, nZ), _oracle_profile(oracle, d, nZ))
+
+
+def test_domains_to_space_errors():
+    with pytest.raises(ValueError):
+        P.domainsToSpace([1.0, 0.0, 2.0], 10)
+    with pytest.raises(ValueError):
+        P.domainsToSpace([1.0, 1e-3, 1e-3, 1.0], 10)
+
+
+def test_periodic_poling_and_round_trip():
+    d = P.periodicPoling(2 * math.pi / 0.1, 1.0)  # domains of 0.05 over a length of 1
+    assert abs(d.sum() - 1.0) < 1e-12 and np.all(d > 0)
+    # integer-aligned domains survive profile -> domains -> profile exactly
+    r = np.random.default_rng(5)
+    steps = r.integers(1, 6, size=12)
+    nZ = int(steps.sum())
+    prof = P.domainsToSpace(steps.astype(float), nZ)
+    assert set(np.unique(prof)) <= {-1.0, 1.0}
+    lengths, first = P.spaceToDomains(prof, float(nZ))
+    assert first == 1.0
+    np.testing.assert_allclose(lengths, steps)
+    np.testing.assert_array_equal(P.domainsToSpace(lengths, nZ), prof)
+    # a continuous profile maps to the domains of its signs
+    cont = np.sin(np.linspace(0.1, 20, 400))
+    lengths, first = P.spaceToDomains(cont, 2.0)
+    assert first == 1.0 and abs(lengths.sum() - 2.0) < 1e-12
+    assert len(lengths) == 1 + int(np.sum(np.sign(cont[1:]) != np.sign(cont[:-1])))
