@@ -320,6 +320,10 @@ PYBIND11_MODULE(nonlinearmedium, m) {
       .value("Constant", nlm::IntensityProfile::Constant)
       .value("GaussianApodization", nlm::IntensityProfile::GaussianApodization)
       .export_values();
+  // the reference's scenario scripts pass plain integers (tests/05 - Hosaka.py:48,
+  // tests/07 - Wasilewski.py:45)
+  py::implicitly_convertible<int, nlm::PulseType>();
+  py::implicitly_convertible<int, nlm::IntensityProfile>();
 
   // _FullyNonlinearMedium, src/nlmModulePy.cpp:140-164
   fnlm.def("batchSignalSimulation",
