@@ -258,6 +258,11 @@ int nlm_gaussian_covariance(int N, const double* C, const double* S, double* Z, 
 /* out = A A^T for a real row-major M x K matrix A (host buffers), e.g. the
  * covariance of a given quadrature matrix (multimode.calcCovarianceMtx). */
 int nlm_gram(int M, int K, const double* A, double* out);
+/* out = A B^T for real row-major A (M x K) and B (Ncol x K), host buffers: the
+ * products of the Bloch-Messiah decomposition of a quadrature matrix
+ * (decompositions.py:250-353 of the reference: symplectic check S^T Omega S,
+ * the polar factor and the right orthogonal factor). */
+int nlm_gemm_nt(int M, int Ncol, int K, const double* A, const double* B, double* out);
 
 /* Standalone transform with the reference's conventions (src/CustomEigenFFT.h):
  * count rows of length n, forward unscaled / inverse scaled by 1/n. */

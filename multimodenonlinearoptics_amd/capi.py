@@ -19,7 +19,7 @@ SYMBOLS = [
     "nlm_multi_create", "nlm_multi_launch", "nlm_multi_sync", "nlm_multi_output", "nlm_multi_classes",
     "nlm_multi_grid", "nlm_multi_destroy", "nlm_batch_forward", "nlm_batch_adjoint",
     "nlm_greens_compose_dev", "nlm_greens_cascade", "nlm_pump_import",
-    "nlm_batch_grids", "nlm_gaussian_covariance", "nlm_gram",
+    "nlm_batch_grids", "nlm_gaussian_covariance", "nlm_gram", "nlm_gemm_nt",
 ]
 
 _lib = None
@@ -60,6 +60,7 @@ def lib():
         L.nlm_greens_compose.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
         L.nlm_gaussian_covariance.argtypes = [C.c_int, vp, vp, vp, vp]
         L.nlm_gram.argtypes = [C.c_int, C.c_int, vp, vp]
+        L.nlm_gemm_nt.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp]
         L.nlm_greens_compose_dev.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib

@@ -220,10 +220,11 @@ __global__ void k_quadrature(const cplx* __restrict__ C, const cplx* __restrict_
   Z[(size_t)(N + r) * W + N + c] = a.re - b.re;  // Re(C - S)
 }
 
-// out = A A^T for a real row-major M x K matrix A: 32 x 32 tiles, four waves of
-// 16 x 16 FP64 MFMAs (operand layout as in k_compose_mfma).
-__global__ void __launch_bounds__(256) k_gram_mfma(const double* __restrict__ A, int M, int K,
-                                                   double* __restrict__ out) {
+// out = A B^T for real row-major A (M x K) and B (Ncol x K): 32 x 32 tiles, four
+// waves of 16 x 16 FP64 MFMAs (operand layout as in k_compose_mfma).  B = A
+// gives the Gram matrix A A^T.
+__global__ void __launch_bounds__(256) k_gemm_nt_mfma(const double* __restrict__ A, const double* __restrict__ B,
+                                                      int M, int Ncol, int K, double* __restrict__ out) {
   __shared__ double Ar[TILE][APAD], Br[TILE][APAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int row0 = blockIdx.y * TILE, col0 = blockIdx.x * TILE;
@@ -233,7 +234,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma(const double* __restrict__ A,
     for (int e = tid; e < TILE * KB; e += 256) {
       const int r = e / KB, k = e % KB, gk = k0 + k;
       Ar[r][k] = (row0 + r < M && gk < K) ? A[(size_t)(row0 + r) * K + gk] : 0.0;
-      Br[r][k] = (col0 + r < M && gk < K) ? A[(size_t)(col0 + r) * K + gk] : 0.0;
+      Br[r][k] = (col0 + r < Ncol && gk < K) ? B[(size_t)(col0 + r) * K + gk] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -247,7 +248,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma(const double* __restrict__ A,
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int orow = row0 + wr + (lane >> 4) + 4 * r;
-    if (orow < M && oc < M) out[(size_t)orow * M + oc] = acc[r];
+    if (orow < M && oc < Ncol) out[(size_t)orow * Ncol + oc] = acc[r];
   }
 }
 
@@ -268,7 +269,7 @@ extern "C" int nlm_gaussian_covariance(int N, const double* C, const double* S, 
   }
   if (err == hipSuccess && cov) {
     const dim3 grid((unsigned)((W + TILE - 1) / TILE), (unsigned)((W + TILE - 1) / TILE));
-    hipLaunchKernelGGL(k_gram_mfma, grid, dim3(256), 0, 0, dZ.p, (int)W, (int)W, dV.p);
+    hipLaunchKernelGGL(k_gemm_nt_mfma, grid, dim3(256), 0, 0, dZ.p, dZ.p, (int)W, (int)W, (int)W, dV.p);
     err = hipGetLastError();
   }
   if (err == hipSuccess && Z) err = hipMemcpy(Z, dZ.p, W * W * sizeof(double), hipMemcpyDeviceToHost);
@@ -277,19 +278,30 @@ extern "C" int nlm_gaussian_covariance(int N, const double* C, const double* S, 
   return NLM_OK;
 }
 
-extern "C" int nlm_gram(int M, int K, const double* A, double* out) {
-  if (M < 0 || K < 0 || (M > 0 && (!A || !out))) return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
-  if (M == 0) return NLM_OK;
-  DevBuf<double> dA, dO;
+extern "C" int nlm_gemm_nt(int M, int Ncol, int K, const double* A, const double* B, double* out) {
+  if (M < 0 || Ncol < 0 || K < 0 || (M > 0 && Ncol > 0 && (!A || !B || !out)))
+    return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
+  if (M == 0 || Ncol == 0) return NLM_OK;
+  const bool same = A == B && M == Ncol;
+  DevBuf<double> dA, dB, dO;
   int e;
-  if ((e = dA.ensure((size_t)M * (K > 0 ? K : 1))) || (e = dO.ensure((size_t)M * M))) return e;
+  if ((e = dA.ensure((size_t)M * (K > 0 ? K : 1))) || (!same && (e = dB.ensure((size_t)Ncol * (K > 0 ? K : 1)))) ||
+      (e = dO.ensure((size_t)M * Ncol)))
+    return e;
   hipError_t err = K > 0 ? hipMemcpy(dA.p, A, (size_t)M * K * sizeof(double), hipMemcpyHostToDevice) : hipSuccess;
+  if (err == hipSuccess && !same && K > 0)
+    err = hipMemcpy(dB.p, B, (size_t)Ncol * K * sizeof(double), hipMemcpyHostToDevice);
   if (err == hipSuccess) {
-    const dim3 grid((unsigned)((M + TILE - 1) / TILE), (unsigned)((M + TILE - 1) / TILE));
-    hipLaunchKernelGGL(k_gram_mfma, grid, dim3(256), 0, 0, dA.p, M, K, dO.p);
+    const dim3 grid((unsigned)((Ncol + TILE - 1) / TILE), (unsigned)((M + TILE - 1) / TILE));
+    hipLaunchKernelGGL(k_gemm_nt_mfma, grid, dim3(256), 0, 0, dA.p, same ? dA.p : dB.p, M, Ncol, K, dO.p);
     err = hipGetLastError();
   }
-  if (err == hipSuccess) err = hipMemcpy(out, dO.p, (size_t)M * M * sizeof(double), hipMemcpyDeviceToHost);
-  if (err != hipSuccess) return set_err(NLM_DEVICE_ERROR, std::string("nlm_gram: ") + hipGetErrorString(err));
+  if (err == hipSuccess) err = hipMemcpy(out, dO.p, (size_t)M * Ncol * sizeof(double), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return set_err(NLM_DEVICE_ERROR, std::string("nlm_gemm_nt: ") + hipGetErrorString(err));
   return NLM_OK;
+}
+
+extern "C" int nlm_gram(int M, int K, const double* A, double* out) {
+  if (M < 0 || K < 0 || (M > 0 && (!A || !out))) return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
+  return nlm_gemm_nt(M, M, K, A, A, out);
 }

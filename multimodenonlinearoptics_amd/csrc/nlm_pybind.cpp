@@ -320,10 +320,11 @@ PYBIND11_MODULE(nonlinearmedium, m) {
       .value("Constant", nlm::IntensityProfile::Constant)
       .value("GaussianApodization", nlm::IntensityProfile::GaussianApodization)
       .export_values();
-  // the reference's scenario scripts pass plain integers (tests/05 - Hosaka.py:48,
-  // tests/07 - Wasilewski.py:45)
+  // the reference's scenario scripts pass pulseType as a plain integer
+  // (tests/05 - Hosaka.py:48, tests/07 - Wasilewski.py:45).  IntensityProfile stays
+  // strict: Chi2SFGII's swapped keyword slots (solvers/Chi2SFGII.cpp:97-106) must
+  // reject the default zPrecision=100 landing in the profile slot.
   py::implicitly_convertible<int, nlm::PulseType>();
-  py::implicitly_convertible<int, nlm::IntensityProfile>();
 
   // _FullyNonlinearMedium, src/nlmModulePy.cpp:140-164
   fnlm.def("batchSignalSimulation",

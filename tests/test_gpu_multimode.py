@@ -27,6 +27,14 @@ def test_quadrature_and_gram_match_numpy(nlm):
         out = np.empty((m, m))
         capi.check(capi.lib().nlm_gram(m, k, A.ctypes.data, out.ctypes.data))
         assert rel_l2(out, A @ A.T) < 1e-13
+    for m, nc, k in [(1, 5, 3), (40, 17, 65), (96, 130, 1), (7, 7, 0)]:
+        A, B = r.standard_normal((m, k)), r.standard_normal((nc, k))
+        out = np.full((m, nc), np.nan)
+        capi.check(capi.lib().nlm_gemm_nt(m, nc, k, A.ctypes.data, B.ctypes.data, out.ctypes.data))
+        if k:
+            assert rel_l2(out, A @ B.T) < 1e-13
+        else:
+            assert not out.any()
 
 
 def test_covariance_of_pdc_greens_is_symplectic(nlm):

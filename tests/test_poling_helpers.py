@@ -46,3 +46,13 @@ def test_periodic_poling_and_round_trip():
     lengths, first = P.spaceToDomains(cont, 2.0)
     assert first == 1.0 and abs(lengths.sum() - 2.0) < 1e-12
     assert len(lengths) == 1 + int(np.sum(np.sign(cont[1:]) != np.sign(cont[:-1])))
+
+
+def test_bloch_messiah_shape_errors():
+    from multimodenonlinearoptics_amd.decompositions import bloch_messiah, sympmat
+    with pytest.raises(ValueError, match="not square"):
+        bloch_messiah(np.zeros((4, 6)))
+    with pytest.raises(ValueError, match="even number"):
+        bloch_messiah(np.eye(5))
+    om = sympmat(2)
+    assert np.array_equal(om @ om, -np.eye(4)) and np.array_equal(om.T, -om)
