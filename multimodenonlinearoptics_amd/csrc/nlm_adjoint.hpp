@@ -77,6 +77,15 @@ struct RegTwiddles {
 // Sum conj(x) h (CONJ) or x h over the workgroup's FA fields and write the
 // total to dst[0..N) (through the fields' transform buffers, which are free
 // between transforms).
+// Partials are read once by another kernel: stream them past the caches.
+NLM_DEV cplx load_stream(const cplx* src) {
+  return mk(__builtin_nontemporal_load(&src->re), __builtin_nontemporal_load(&src->im));
+}
+NLM_DEV void store_stream(cplx* dst, cplx v) {
+  __builtin_nontemporal_store(v.re, &dst->re);
+  __builtin_nontemporal_store(v.im, &dst->im);
+}
+
 template <int L, int FA, bool CONJ>
 NLM_DEV void wg_sum_product(const cplx (&x)[8], const cplx (&h)[8], bool active, cplx* lds, int lf, int t,
                             cplx* __restrict__ dst) {
@@ -84,7 +93,7 @@ NLM_DEV void wg_sum_product(const cplx (&x)[8], const cplx (&h)[8], bool active,
   constexpr int N = F::N, T = F::T, E = F::lds_elems;
   if constexpr (FA == 1) {
 #pragma unroll
-    for (int m = 0; m < 8; m++) dst[t + T * m] = active ? (CONJ ? conj(x[m]) : x[m]) * h[m] : mk(0, 0);
+    for (int m = 0; m < 8; m++) store_stream(dst + t + T * m, active ? (CONJ ? conj(x[m]) : x[m]) * h[m] : mk(0, 0));
   } else {
     lds_barrier();
 #pragma unroll
@@ -96,8 +105,37 @@ NLM_DEV void wg_sum_product(const cplx (&x)[8], const cplx (&h)[8], bool active,
       cplx s = lds[lpad(k)];
 #pragma unroll
       for (int f = 1; f < FA; f++) s = s + lds[f * E + lpad(k)];
-      dst[k] = s;
+      store_stream(dst + k, s);
     }
+  }
+}
+
+// Both table sums at once, g_alpha partial = sum conj(x) h to dst[0..N) and
+// g_beta partial = sum x h to dst[N..2N), through 2 FA transform buffers (two
+// barriers instead of four).
+template <int L, int FA>
+NLM_DEV void wg_sum_products2(const cplx (&x)[8], const cplx (&h)[8], bool active, cplx* lds, int lf, int t,
+                              cplx* __restrict__ dst) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T, E = F::lds_elems;
+  lds_barrier();
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    lds[lf * E + lpad(t + T * m)] = active ? conj(x[m]) * h[m] : mk(0, 0);
+    lds[(FA + lf) * E + lpad(t + T * m)] = active ? x[m] * h[m] : mk(0, 0);
+  }
+  lds_barrier();
+#pragma unroll
+  for (int j = 0; j < 8 / FA; j++) {
+    const int k = lf * T + t + FA * T * j;
+    cplx sa = lds[lpad(k)], sb = lds[FA * E + lpad(k)];
+#pragma unroll
+    for (int f = 1; f < FA; f++) {
+      sa = sa + lds[f * E + lpad(k)];
+      sb = sb + lds[(FA + f) * E + lpad(k)];
+    }
+    store_stream(dst + k, sa);
+    store_stream(dst + N + k, sb);
   }
 }
 
@@ -119,7 +157,13 @@ NLM_DEV void inverse_step(cplx (&x)[8], const cplx (&r)[8], const cplx (&a)[8], 
 #pragma unroll
   for (int m = 0; m < 8; m++) {
     const double den = N * (abs2(a[m]) - abs2(b[m]));
-    x[m] = (conj(a[m]) * r[m] - b[m] * conj(r[m])) * (1.0 / den);
+    // hardware reciprocal refined by two Newton steps (relative error ~1 ulp;
+    // den is |alpha|^2 - |beta|^2 of a step close to the identity, far from 0
+    // and from the denormal range)
+    double rd = __builtin_amdgcn_rcp(den);
+    rd = rd * __builtin_fma(-den, rd, 2.0);
+    rd = rd * __builtin_fma(-den, rd, 2.0);
+    x[m] = (conj(a[m]) * r[m] - b[m] * conj(r[m])) * rd;
   }
 }
 
@@ -223,8 +267,12 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
     // adjoint of r = alpha x + beta conj(x), h = g: g_alpha += conj(x) h,
     // g_beta += x h (workgroup sums), g_x = conj(alpha) h + beta conj(h)
     cplx* dst = ap.part + (((size_t)blockIdx.x * ap.chunk + (i - ap.iLo - 1)) * 2) * N;
-    wg_sum_product<L, FA, true>(x, g, active, lds, lf, t, dst);
-    wg_sum_product<L, FA, false>(x, g, active, lds, lf, t, dst + N);
+    if constexpr (FA > 1 && FUSE) {
+      wg_sum_products2<L, FA>(x, g, active, lds, lf, t, dst);
+    } else {
+      wg_sum_product<L, FA, true>(x, g, active, lds, lf, t, dst);
+      wg_sum_product<L, FA, false>(x, g, active, lds, lf, t, dst + N);
+    }
 #pragma unroll
     for (int m = 0; m < 8; m++) {
       const cplx h = g[m], a = al[m] * (double)N, b = be[m] * (double)N;
@@ -272,6 +320,7 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
 // One block per (step, 64 time points): wave q sums workgroups
 // q, q + RED_WAVES, ... (fixed order), then lane tau adds the wave sums in order.
 constexpr int RED_WAVES = 8;
+constexpr int RED_ACC = 4;
 __global__ void __launch_bounds__(64 * RED_WAVES) k_adj_reduce(const cplx* __restrict__ part, int nWG, int chunk,
                                                                int iLo, int N, const cplx* __restrict__ coef,
                                                                const cplx* __restrict__ pump, cplx* __restrict__ gm) {
@@ -281,11 +330,36 @@ __global__ void __launch_bounds__(64 * RED_WAVES) k_adj_reduce(const cplx* __res
   const int q = threadIdx.x / 64, lane = threadIdx.x % 64, k = k0 + lane;
   cplx ga = mk(0, 0), gb = mk(0, 0);
   if (k < N) {
-    for (int w = q; w < nWG; w += RED_WAVES) {
-      const cplx* p = part + (((size_t)w * chunk + jj) * 2) * N;
-      ga = ga + p[k];
-      gb = gb + p[N + k];
+    // RED_ACC independent accumulators (workgroups w + RED_WAVES u of each group
+    // of RED_ACC RED_WAVES) keep that many loads in flight per lane; the
+    // summation order is fixed
+    cplx a4[RED_ACC], b4[RED_ACC];
+#pragma unroll
+    for (int u = 0; u < RED_ACC; u++) a4[u] = b4[u] = ga;
+    const size_t stride = (size_t)RED_WAVES * chunk * 2 * N;
+    const cplx* p = part + (((size_t)q * chunk + jj) * 2) * N + k;
+    int w = q;
+    for (; w + (RED_ACC - 1) * RED_WAVES < nWG; w += RED_ACC * RED_WAVES, p += RED_ACC * stride) {
+#pragma unroll
+      for (int u = 0; u < RED_ACC; u++) {
+        a4[u] = a4[u] + load_stream(p + u * stride);
+        b4[u] = b4[u] + load_stream(p + u * stride + N);
+      }
     }
+    for (int u = 0; w < nWG; w += RED_WAVES, p += stride, u++) {
+      a4[u] = a4[u] + load_stream(p);
+      b4[u] = b4[u] + load_stream(p + N);
+    }
+#pragma unroll
+    for (int h = RED_ACC / 2; h >= 1; h /= 2) {
+#pragma unroll
+      for (int u = 0; u < h; u++) {
+        a4[u] = a4[u] + a4[u + h];
+        b4[u] = b4[u] + b4[u + h];
+      }
+    }
+    ga = a4[0];
+    gb = b4[0];
   }
   red[0][q][lane] = ga;
   red[1][q][lane] = gb;
