@@ -77,14 +77,6 @@ struct RegTwiddles {
 // Sum conj(x) h (CONJ) or x h over the workgroup's FA fields and write the
 // total to dst[0..N) (through the fields' transform buffers, which are free
 // between transforms).
-// Partials are read once by another kernel: stream them past the caches.
-NLM_DEV cplx load_stream(const cplx* src) {
-  return mk(__builtin_nontemporal_load(&src->re), __builtin_nontemporal_load(&src->im));
-}
-NLM_DEV void store_stream(cplx* dst, cplx v) {
-  __builtin_nontemporal_store(v.re, &dst->re);
-  __builtin_nontemporal_store(v.im, &dst->im);
-}
 
 template <int L, int FA, bool CONJ>
 NLM_DEV void wg_sum_product(const cplx (&x)[8], const cplx (&h)[8], bool active, cplx* lds, int lf, int t,

@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(MIX_THREADS) k_big_mix(BigParams bp, int mode,
 #pragma unroll
       for (int n1 = 0; n1 < P; n1++) {
         const size_t q = fo + n1 * M + n2;
-        if (keep) seg_slot(bp, i, 4 + st)[q] = v[n1];
+        if (keep) store_stream(seg_slot(bp, i, 4 + st) + q, v[n1]);
         u[n1] = v[n1] * bp.Y[q];
       }
       break;
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(MIX_THREADS) k_big_mix(BigParams bp, int mode,
         bp.ACC[q] = s == 0 ? v[n1] : bp.ACC[q] + 2.0 * v[n1];
         const cplx y = bp.A[q] + h * v[n1];
         bp.Y[q] = y;
-        if (keep) seg_slot(bp, i, 1 + s)[q] = y;
+        if (keep) store_stream(seg_slot(bp, i, 1 + s) + q, y);
         u[n1] = mk(r * abs2(y), 0.0);
       }
       break;
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(MIX_THREADS) k_big_mix(BigParams bp, int mode,
       }
       if (kind == MIX_DISP && bp.ckptWrite && i % bp.ckptEvery == 0) {
 #pragma unroll
-        for (int n1 = 0; n1 < P; n1++) bp.ckpt[((size_t)f * bp.nSeg + i / bp.ckptEvery) * N + n1 * M + n2] = v[n1];
+        for (int n1 = 0; n1 < P; n1++) store_stream(bp.ckpt + ((size_t)f * bp.nSeg + i / bp.ckptEvery) * N + n1 * M + n2, v[n1]);
       }
       const double r = stage_r(bp, i + 1, 0);
       const bool keep = seg_has(bp, i + 1);
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(MIX_THREADS) k_big_mix(BigParams bp, int mode,
         const size_t q = fo + n1 * M + n2;
         bp.A[q] = v[n1];
         bp.Y[q] = v[n1];
-        if (keep) seg_slot(bp, i + 1, 0)[q] = v[n1];
+        if (keep) store_stream(seg_slot(bp, i + 1, 0) + q, v[n1]);
         u[n1] = mk(r * abs2(v[n1]), 0.0);
       }
       break;
@@ -235,8 +235,8 @@ __global__ void __launch_bounds__(MIX_THREADS) k_big_mix(BigParams bp, int mode,
 #pragma unroll
       for (int n1 = 0; n1 < P; n1++) {
         const size_t q = fo + n1 * M + n2;
-        bp.TMP[q] = conj(vs[q]) * v[n1];
-        u[n1] = conj(ys[q]) * v[n1];
+        bp.TMP[q] = conj(load_stream(vs + q)) * v[n1];  // last read of v_s
+        u[n1] = conj(ys[q]) * v[n1];                     // y_s is read again by MIX_ADJ_S
       }
       break;
     }
@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(MIX_THREADS) k_big_mix(BigParams bp, int mode,
 #pragma unroll
       for (int n1 = 0; n1 < P; n1++) {
         const size_t q = fo + n1 * M + n2;
-        const cplx gy = bp.TMP[q] + (r2 * v[n1].re) * ys[q];
+        const cplx gy = bp.TMP[q] + (r2 * v[n1].re) * load_stream(ys + q);  // last read of y_s
         const cplx ga = bp.GA[q] + gy;
         bp.GA[q] = ga;
         if (st == 0) {

@@ -35,6 +35,17 @@ __host__ __device__ __forceinline__ cplx operator*(double s, cplx a) { return cp
 __host__ __device__ __forceinline__ cplx operator*(cplx a, double s) { return cplx{a.re * s, a.im * s}; }
 __host__ __device__ __forceinline__ cplx conj(cplx a) { return cplx{a.re, -a.im}; }
 __host__ __device__ __forceinline__ double abs2(cplx a) { return a.re * a.re + a.im * a.im; }
+
+// Loads and stores of data that is touched once (workgroup partials, stored
+// stage data of a checkpoint segment): non-temporal, so that it streams past
+// L2 and the Infinity Cache instead of evicting the working set.
+__device__ __forceinline__ cplx load_stream(const cplx* src) {
+  return mk(__builtin_nontemporal_load(&src->re), __builtin_nontemporal_load(&src->im));
+}
+__device__ __forceinline__ void store_stream(cplx* dst, cplx v) {
+  __builtin_nontemporal_store(v.re, &dst->re);
+  __builtin_nontemporal_store(v.im, &dst->im);
+}
 // multiply by -i (forward) or +i (inverse)
 template <bool INV>
 __device__ __forceinline__ cplx mul_mi(cplx a) {
