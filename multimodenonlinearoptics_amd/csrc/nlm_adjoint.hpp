@@ -89,14 +89,14 @@ NLM_DEV void wg_sum_product(const cplx (&x)[8], const cplx (&h)[8], bool active,
   } else {
     lds_barrier();
 #pragma unroll
-    for (int m = 0; m < 8; m++) lds[lf * E + lpad(t + T * m)] = active ? (CONJ ? conj(x[m]) : x[m]) * h[m] : mk(0, 0);
+    for (int m = 0; m < 8; m++) lds[lf * E + lpad<L>(t + T * m)] = active ? (CONJ ? conj(x[m]) : x[m]) * h[m] : mk(0, 0);
     lds_barrier();
 #pragma unroll
     for (int j = 0; j < 8 / FA; j++) {
       const int k = lf * T + t + FA * T * j;
-      cplx s = lds[lpad(k)];
+      cplx s = lds[lpad<L>(k)];
 #pragma unroll
-      for (int f = 1; f < FA; f++) s = s + lds[f * E + lpad(k)];
+      for (int f = 1; f < FA; f++) s = s + lds[f * E + lpad<L>(k)];
       store_stream(dst + k, s);
     }
   }
@@ -113,18 +113,18 @@ NLM_DEV void wg_sum_products2(const cplx (&x)[8], const cplx (&h)[8], bool activ
   lds_barrier();
 #pragma unroll
   for (int m = 0; m < 8; m++) {
-    lds[lf * E + lpad(t + T * m)] = active ? conj(x[m]) * h[m] : mk(0, 0);
-    lds[(FA + lf) * E + lpad(t + T * m)] = active ? x[m] * h[m] : mk(0, 0);
+    lds[lf * E + lpad<L>(t + T * m)] = active ? conj(x[m]) * h[m] : mk(0, 0);
+    lds[(FA + lf) * E + lpad<L>(t + T * m)] = active ? x[m] * h[m] : mk(0, 0);
   }
   lds_barrier();
 #pragma unroll
   for (int j = 0; j < 8 / FA; j++) {
     const int k = lf * T + t + FA * T * j;
-    cplx sa = lds[lpad(k)], sb = lds[FA * E + lpad(k)];
+    cplx sa = lds[lpad<L>(k)], sb = lds[FA * E + lpad<L>(k)];
 #pragma unroll
     for (int f = 1; f < FA; f++) {
-      sa = sa + lds[f * E + lpad(k)];
-      sb = sb + lds[(FA + f) * E + lpad(k)];
+      sa = sa + lds[f * E + lpad<L>(k)];
+      sb = sb + lds[(FA + f) * E + lpad<L>(k)];
     }
     store_stream(dst + k, sa);
     store_stream(dst + N + k, sb);

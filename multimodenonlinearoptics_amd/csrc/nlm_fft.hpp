@@ -63,10 +63,15 @@ struct FftShape {
     for (int i = 0; i < s; i++) p *= radix(i);
     return p;
   }
-  static constexpr int lds_elems = N + N / 8;            // padded LDS footprint (cplx)
+  // LDS padding: one slot after every 2^PAD elements.  Measured on the step
+  // kernels (N = 512: 5.6 % faster with PAD 5 than 3; N = 1024: the adjoint
+  // sweep is 4 % slower with 5, PAD 2 and 4 slower everywhere).
+  static constexpr int PAD = L <= 9 ? 5 : 3;
+  static constexpr int lds_elems = N + (N >> PAD);       // padded LDS footprint (cplx)
 };
 
-__device__ __forceinline__ int lpad(int x) { return x + (x >> 3); }
+template <int L>
+__device__ __forceinline__ int lpad(int x) { return x + (x >> FftShape<L>::PAD); }
 
 // sin and cos of a large argument (dispersion phases reach 1e7 rad) to about
 // one ulp: Cody-Waite reduction by pi/2 split into two doubles, each
@@ -184,7 +189,7 @@ __device__ __forceinline__ void fft_stages(cplx (&v)[8], cplx* __restrict__ buf,
     const int beta = t + T * j;
     const int b = beta % M, g = beta / M;
 #pragma unroll
-    for (int a = 0; a < R; a++) v[j * R + a] = buf[lpad(g * Mp + a * M + b)];
+    for (int a = 0; a < R; a++) v[j * R + a] = buf[lpad<L>(g * Mp + a * M + b)];
   }
 #pragma unroll
   for (int j = 0; j < NB; j++) dft_small<R, INV>(v + j * R);
@@ -207,7 +212,7 @@ __device__ __forceinline__ void fft_stages(cplx (&v)[8], cplx* __restrict__ buf,
 #pragma unroll
       for (int c = 0; c < R; c++) {
         cplx x = c == 0 ? v[j * R] : twmul<INV>(v[j * R + c], tw.w[s][j * R + c]);
-        buf[lpad((g + P * c) * M + b)] = x;
+        buf[lpad<L>((g + P * c) * M + b)] = x;
       }
     }
     fft_stages<L, INV, s + 1>(v, buf, t, tw);
@@ -227,7 +232,7 @@ __device__ __forceinline__ void fft_field(cplx (&v)[8], cplx* __restrict__ buf, 
     for (int c = 1; c < 8; c++) v[c] = twmul<INV>(v[c], tw.w[0][c]);
     lds_barrier();  // previous readers of buf are done
 #pragma unroll
-    for (int c = 0; c < 8; c++) buf[lpad(c * T + t)] = v[c];
+    for (int c = 0; c < 8; c++) buf[lpad<L>(c * T + t)] = v[c];
     fft_stages<L, INV, 1>(v, buf, t, tw);
   }
 }
@@ -249,8 +254,8 @@ __device__ __forceinline__ void fft_stages2(cplx (&v)[8], cplx (&u)[8], cplx* __
     const int b = beta % M, g = beta / M;
 #pragma unroll
     for (int a = 0; a < R; a++) {
-      v[j * R + a] = bv[lpad(g * Mp + a * M + b)];
-      u[j * R + a] = bu[lpad(g * Mp + a * M + b)];
+      v[j * R + a] = bv[lpad<L>(g * Mp + a * M + b)];
+      u[j * R + a] = bu[lpad<L>(g * Mp + a * M + b)];
     }
   }
 #pragma unroll
@@ -283,12 +288,12 @@ __device__ __forceinline__ void fft_stages2(cplx (&v)[8], cplx (&u)[8], cplx* __
 #pragma unroll
       for (int c = 0; c < R; c++) {
         if (c == 0) {
-          bv[lpad(g * M + b)] = v[j * R];
-          bu[lpad(g * M + b)] = u[j * R];
+          bv[lpad<L>(g * M + b)] = v[j * R];
+          bu[lpad<L>(g * M + b)] = u[j * R];
         } else {
           const cplx w = tw(s, j * R + c);
-          bv[lpad((g + P * c) * M + b)] = twmul<INV>(v[j * R + c], w);
-          bu[lpad((g + P * c) * M + b)] = twmul<INV>(u[j * R + c], w);
+          bv[lpad<L>((g + P * c) * M + b)] = twmul<INV>(v[j * R + c], w);
+          bu[lpad<L>((g + P * c) * M + b)] = twmul<INV>(u[j * R + c], w);
         }
       }
     }
@@ -313,8 +318,8 @@ __device__ __forceinline__ void fft_field2(cplx (&v)[8], cplx (&u)[8], cplx* __r
     lds_barrier();
 #pragma unroll
     for (int c = 0; c < 8; c++) {
-      bv[lpad(c * T + t)] = v[c];
-      bu[lpad(c * T + t)] = u[c];
+      bv[lpad<L>(c * T + t)] = v[c];
+      bu[lpad<L>(c * T + t)] = u[c];
     }
     fft_stages2<L, INV, 1>(v, u, bv, bu, t, tw);
   }
@@ -332,7 +337,7 @@ __device__ __forceinline__ void fft_stages1(cplx (&v)[8], cplx* __restrict__ buf
     const int beta = t + T * j;
     const int b = beta % M, g = beta / M;
 #pragma unroll
-    for (int a = 0; a < R; a++) v[j * R + a] = buf[lpad(g * Mp + a * M + b)];
+    for (int a = 0; a < R; a++) v[j * R + a] = buf[lpad<L>(g * Mp + a * M + b)];
   }
 #pragma unroll
   for (int j = 0; j < NB; j++) dft_small<R, INV>(v + j * R);
@@ -354,7 +359,7 @@ __device__ __forceinline__ void fft_stages1(cplx (&v)[8], cplx* __restrict__ buf
       const int b = beta % M, g = beta / M;
 #pragma unroll
       for (int c = 0; c < R; c++)
-        buf[lpad((g + P * c) * M + b)] = c == 0 ? v[j * R] : twmul<INV>(v[j * R + c], tw(s, j * R + c));
+        buf[lpad<L>((g + P * c) * M + b)] = c == 0 ? v[j * R] : twmul<INV>(v[j * R + c], tw(s, j * R + c));
     }
     fft_stages1<L, INV, s + 1>(v, buf, t, tw);
   }
@@ -370,7 +375,7 @@ __device__ __forceinline__ void fft_field1(cplx (&v)[8], cplx* __restrict__ buf,
     for (int c = 1; c < 8; c++) v[c] = twmul<INV>(v[c], tw(0, c));
     lds_barrier();
 #pragma unroll
-    for (int c = 0; c < 8; c++) buf[lpad(c * T + t)] = v[c];
+    for (int c = 0; c < 8; c++) buf[lpad<L>(c * T + t)] = v[c];
     fft_stages1<L, INV, 1>(v, buf, t, tw);
   }
 }
