@@ -67,13 +67,6 @@ struct AdjShape {
   static constexpr int LDS_ELEMS = BUFS * FftShape<L>::lds_elems;
 };
 
-// Register twiddles behind an accessor tw(stage, entry) (fft_field1 / fft_field2).
-template <int L>
-struct RegTwiddles {
-  FftTwiddles<L> w;
-  __device__ __forceinline__ cplx operator()(int s, int idx) const { return w.w[s][idx]; }
-};
-
 // Sum conj(x) h (CONJ) or x h over the workgroup's FA fields and write the
 // total to dst[0..N) (through the fields' transform buffers, which are free
 // between transforms).

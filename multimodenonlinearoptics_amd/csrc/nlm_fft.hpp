@@ -19,6 +19,7 @@
 // forward X_k = sum_j x_j exp(-2 pi i jk/N) unscaled; the inverse uses
 // exp(+2 pi i jk/N) and its 1/N factor is applied by the caller.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 struct __align__(16) cplx {
@@ -175,6 +176,66 @@ __device__ __forceinline__ void fft_load_twiddles(FftTwiddles<L>& tw, const cplx
       for (int c = 1; c < R; c++) tw.w[s][j * R + c] = table[(b * c * P) & (N - 1)];
     }
   }
+}
+
+// Twiddle sources for the accessor-based transforms (fft_field1 / fft_field2):
+// tw(stage, entry) returns FftTwiddles<L>::w[stage][entry] of this lane.
+//  RegTwiddles:   the lane's twiddles held in registers (24 complex at most);
+//  TableTwiddles: looked up in the N-entry table on each use, in LDS or in global
+//                 memory, for fields whose lanes cannot spare the registers.
+template <int L>
+struct RegTwiddles {
+  FftTwiddles<L> w;
+  __device__ __forceinline__ cplx operator()(int s, int idx) const { return w.w[s][idx]; }
+};
+
+template <int L>
+struct TableTwiddles {
+  const cplx* tab;
+  int t;
+  __device__ __forceinline__ cplx operator()(int s, int idx) const {
+    using F = FftShape<L>;
+    const int R = F::radix(s), P = F::before(s), M = F::N / P / R;
+    const int j = idx / R, c = idx % R;
+    const int b = (t + F::T * j) % M;
+    return tab[(b * c * P) & (F::N - 1)];
+  }
+};
+
+// Twiddles of the propagation kernels.  Up to N = 2048 a field's lanes have 512
+// registers each (one wave per SIMD) and keep their twiddles in registers.  From
+// N = 4096 a field needs 512 or 1024 lanes, i.e. two or four waves per SIMD with
+// 256 or 128 registers: the twiddles are looked up in the table instead, an LDS
+// copy beside the field's buffer at N = 4096, the global table (through L1/L2)
+// at N = 8192, whose buffer fills most of the LDS.  (Measured on a batch of 2000
+// SHG / PDC media: the N = 4096 Chi2SHG instances 13 % and the longest Chi2PDC
+// instance 10 % faster together with the dispersion row read where it is used;
+// the same from N = 2048 was 1 % slower.)
+template <int L>
+struct PropTw {
+  static constexpr bool LEAN = L >= 12;
+  static constexpr bool IN_LDS = L == 12;
+  static constexpr int LDS_ELEMS = IN_LDS ? (1 << L) : 1;
+  using type = typename std::conditional<LEAN, TableTwiddles<L>, RegTwiddles<L>>::type;
+};
+
+// Fill the workgroup's LDS copy of the twiddle table (all threads; ends with a barrier).
+template <int L>
+__device__ __forceinline__ void fill_twiddle_lds(cplx* lds_tab, const cplx* table) {
+  if constexpr (PropTw<L>::IN_LDS) {
+    for (int k = threadIdx.x; k < (1 << L); k += blockDim.x) lds_tab[k] = table[k];
+    __syncthreads();
+  }
+}
+
+template <int L>
+__device__ __forceinline__ void init_twiddles(RegTwiddles<L>& tw, const cplx* table, const cplx*, int t) {
+  fft_load_twiddles<L>(tw.w, table, t);
+}
+template <int L>
+__device__ __forceinline__ void init_twiddles(TableTwiddles<L>& tw, const cplx* table, const cplx* lds_tab, int t) {
+  tw.tab = PropTw<L>::IN_LDS ? lds_tab : table;
+  tw.t = t;
 }
 
 // Stages s >= 1 (recursive template so every index is a compile-time shape).

@@ -53,8 +53,10 @@ __global__ void __launch_bounds__(MultiLaunch<L>::THREADS)
                  const cplx* tw) {
   using F = FftShape<L>;
   __shared__ cplx lds[F::lds_elems];
+  __shared__ cplx twl[PropTw<L>::LDS_ELEMS];
   __shared__ int item;
   const int t = threadIdx.x;
+  fill_twiddle_lds<L>(twl, tw);
   for (;;) {
     __syncthreads();  // the previous instance's readers of `item` are done
     if (t == 0) item = atomicAdd(counter, 1);
@@ -86,7 +88,7 @@ __global__ void __launch_bounds__(MultiLaunch<L>::THREADS)
     kp.Dp = I.Dp;
     kp.strength = I.strength;
     kp.dzp = I.dzp;
-    prop_field<M, L, INLINE>(kp, 0, true, t, lds);
+    prop_field<M, L, INLINE>(kp, 0, true, t, lds, twl);
   }
 }
 

@@ -145,13 +145,13 @@ __device__ __forceinline__ void load_inputs(cplx (&a)[NS][8], const KParams& kp,
 
 // Half dispersion step in: a <- FFT(a) (time-domain input) * exp(i dz/2 D);
 // the grid path records row 0 of the frequency grid.
-template <int L, int NS>
+template <int L, int NS, class TW>
 __device__ __forceinline__ void prologue(cplx (&a)[NS][8], const KParams& kp, cplx* buf, int t,
-                                         const FftTwiddles<L>& tw, int field, bool active) {
+                                         const TW& tw, int field, bool active) {
   constexpr int N = FftShape<L>::N;
 #pragma unroll
   for (int q = 0; q < NS; q++) {
-    if (kp.inTime) fft_field<L, false>(a[q], buf, t, tw);
+    if (kp.inTime) fft_field1<L, false>(a[q], buf, t, tw);
     mul_table<L>(a[q], kp.halfIn + (size_t)q * N, t);
     if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, NS, q, 0, N), a[q], t, active);
   }
@@ -159,9 +159,9 @@ __device__ __forceinline__ void prologue(cplx (&a)[NS][8], const KParams& kp, cp
 
 // Final half step out and output write (src/_NonlinearMedium.hpp:235-238,
 // src/_NonlinearMedium.cpp:333-345, 428-431).  `a` holds the last frequency row.
-template <int L, int NS>
+template <int L, int NS, class TW>
 __device__ __forceinline__ void epilogue(cplx (&a)[NS][8], const KParams& kp, cplx* buf, int t,
-                                         const FftTwiddles<L>& tw, int field, bool active) {
+                                         const TW& tw, int field, bool active) {
   constexpr int N = FftShape<L>::N;
   const int last = kp.nZ - 1;
 #pragma unroll
@@ -169,13 +169,13 @@ __device__ __forceinline__ void epilogue(cplx (&a)[NS][8], const KParams& kp, cp
     mul_table<L>(a[q], kp.halfOut + (size_t)q * N, t);
     if (kp.kind == KIND_GRID) {
       store_row<L>(grid_row(kp, kp.gridF, field, NS, q, last, N), a[q], t, active);
-      fft_field<L, true>(a[q], buf, t, tw);
+      fft_field1<L, true>(a[q], buf, t, tw);
       scale_inv<L>(a[q]);
       store_row<L>(grid_row(kp, kp.gridT, field, NS, q, last, N), a[q], t, active);
     } else {
       const int op = kp.outPos[q];
       if (kp.outTime) {
-        fft_field<L, true>(a[q], buf, t, tw);
+        fft_field1<L, true>(a[q], buf, t, tw);
         scale_inv<L>(a[q]);
       }
       if (op >= 0) store_row<L>(kp.out + (size_t)field * kp.nOut * N + (size_t)op * N, a[q], t, active);
@@ -196,7 +196,8 @@ template <int L>
 struct InlinePump {
   cplx F0[8], ph[8], e1[8];
   int row;
-  NLM_DEV void init(const KParams& kp, cplx* buf, int t, const FftTwiddles<L>& tw, cplx (&row0)[8]) {
+  template <class TW>
+  NLM_DEV void init(const KParams& kp, cplx* buf, int t, const TW& tw, cplx (&row0)[8]) {
     constexpr int T = FftShape<L>::T;
     const double s0 = kp.strength ? kp.strength[0] : 1.0;
 #pragma unroll
@@ -208,10 +209,11 @@ struct InlinePump {
       e1[m] = mk(cs, sn);
       ph[m] = mk(1.0, 0.0);
     }
-    fft_field<L, false>(F0, buf, t, tw);
+    fft_field1<L, false>(F0, buf, t, tw);
     row = 0;
   }
-  NLM_DEV void next(const KParams& kp, cplx* buf, int t, const FftTwiddles<L>& tw, cplx (&out)[8]) {
+  template <class TW>
+  NLM_DEV void next(const KParams& kp, cplx* buf, int t, const TW& tw, cplx (&out)[8]) {
     constexpr int T = FftShape<L>::T;
     row++;
     const bool exact = (row & 63) == 0;
@@ -227,7 +229,7 @@ struct InlinePump {
       }
       out[m] = F0[m] * ph[m];
     }
-    fft_field<L, true>(out, buf, t, tw);
+    fft_field1<L, true>(out, buf, t, tw);
     const double s = (kp.strength ? kp.strength[row] : 1.0) * (1.0 / FftShape<L>::N);
 #pragma unroll
     for (int m = 0; m < 8; m++) out[m] = out[m] * s;
@@ -238,19 +240,26 @@ struct InlinePump {
 // the T lanes that hold it; INLINE generates the pump rows instead of reading
 // the pump grid.
 template <class M, int L, bool INLINE = false>
-__device__ __forceinline__ void prop_field(const KParams& kp, int field, bool active, int t, cplx* buf) {
+__device__ __forceinline__ void prop_field(const KParams& kp, int field, bool active, int t, cplx* buf,
+                                           const cplx* twl) {
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T, NS = M::NS, NP = M::NP, NC = M::NC;
   static_assert(!INLINE || NP == 1, "the generated pump is a single linear pump");
+  constexpr bool LEAN = PropTw<L>::LEAN;
 
-  FftTwiddles<L> tw;
-  fft_load_twiddles<L>(tw, kp.tw, t);
+  typename PropTw<L>::type tw;
+  init_twiddles<L>(tw, kp.tw, twl, t);
 
-  cplx ds[NS][8];  // exp(i dz D_s) at this lane's frequencies, reused every step
+  // exp(i dz D_s) at this lane's frequencies, reused every step (read from the
+  // cached row at each use from N = 4096)
+  constexpr int DS = LEAN ? 1 : 8;
+  cplx ds[NS][DS];
+  if constexpr (!LEAN) {
 #pragma unroll
-  for (int q = 0; q < NS; q++)
+    for (int q = 0; q < NS; q++)
 #pragma unroll
-    for (int m = 0; m < 8; m++) ds[q][m] = kp.dstep[(size_t)q * N + t + T * m];
+      for (int m = 0; m < 8; m++) ds[q][m] = kp.dstep[(size_t)q * N + t + T * m];
+  }
 
   cplx a[NS][8];
   load_inputs<L, NS>(a, kp, field, t, active);
@@ -258,7 +267,7 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
   if (kp.nZ > 1) {
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-      fft_field<L, true>(a[q], buf, t, tw);
+      fft_field1<L, true>(a[q], buf, t, tw);
       scale_inv<L>(a[q]);
       if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, NS, q, 0, N), a[q], t, active);
       if (kp.ckpt) store_row<L>(kp.ckpt + ((size_t)field * kp.nSeg * NS + q) * N, a[q], t, active);
@@ -333,12 +342,13 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
     // --- dispersion step (src/_NonlinearMedium.hpp:212-215)
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-      fft_field<L, false>(a[q], buf, t, tw);
+      fft_field1<L, false>(a[q], buf, t, tw);
 #pragma unroll
-      for (int m = 0; m < 8; m++) a[q][m] = a[q][m] * ds[q][m];
+      for (int m = 0; m < 8; m++)
+        a[q][m] = a[q][m] * (LEAN ? kp.dstep[(size_t)q * N + t + T * m] : ds[q][LEAN ? 0 : m]);
       if (i < kp.nZ - 1) {
         if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, NS, q, i, N), a[q], t, active);
-        fft_field<L, true>(a[q], buf, t, tw);
+        fft_field1<L, true>(a[q], buf, t, tw);
         scale_inv<L>(a[q]);
         if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, NS, q, i, N), a[q], t, active);
         if (kp.ckpt && i % kp.ckptEvery == 0)
@@ -354,9 +364,11 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop(KParams kp) {
   using F = FftShape<L>;
   constexpr int T = F::T, FPB = Launch<L>::FPB;
   __shared__ cplx lds[FPB * F::lds_elems];
+  __shared__ cplx twl[PropTw<L>::LDS_ELEMS];
+  fill_twiddle_lds<L>(twl, kp.tw);
   const int lf = threadIdx.x / T, t = threadIdx.x % T;
   const int field = blockIdx.x * FPB + lf;
-  prop_field<M, L>(kp, field, field < kp.nFields, t, lds + lf * F::lds_elems);
+  prop_field<M, L>(kp, field, field < kp.nFields, t, lds + lf * F::lds_elems, twl);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,22 +376,22 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop(KParams kp) {
 // ---------------------------------------------------------------------------
 
 // k = IFFT( FFT( IFFT(FFT(r |y|^2) * R) * y ) * (nl0 + nl1*omega) )
-template <int L>
+template <int L, class TW>
 __device__ __forceinline__ void gnlse_stage(const cplx (&y)[8], double r, cplx (&k)[8], const cplx (&ram)[8],
-                                            const cplx (&flt)[8], cplx* buf, int t, const FftTwiddles<L>& tw) {
+                                            const cplx (&flt)[8], cplx* buf, int t, const TW& tw) {
 #pragma unroll
   for (int m = 0; m < 8; m++) k[m] = mk(r * abs2(y[m]), 0.0);
-  fft_field<L, false>(k, buf, t, tw);
+  fft_field1<L, false>(k, buf, t, tw);
 #pragma unroll
   for (int m = 0; m < 8; m++) k[m] = k[m] * ram[m];
-  fft_field<L, true>(k, buf, t, tw);
+  fft_field1<L, true>(k, buf, t, tw);
   scale_inv<L>(k);
 #pragma unroll
   for (int m = 0; m < 8; m++) k[m] = k[m] * y[m];
-  fft_field<L, false>(k, buf, t, tw);
+  fft_field1<L, false>(k, buf, t, tw);
 #pragma unroll
   for (int m = 0; m < 8; m++) k[m] = k[m] * flt[m];
-  fft_field<L, true>(k, buf, t, tw);
+  fft_field1<L, true>(k, buf, t, tw);
   scale_inv<L>(k);
 }
 
@@ -389,14 +401,16 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_gnlse(KParams kp) {
   constexpr int N = F::N, T = F::T;
   constexpr int FPB = Launch<L>::FPB;
   __shared__ cplx lds[FPB * F::lds_elems];
+  __shared__ cplx twl[PropTw<L>::LDS_ELEMS];
+  fill_twiddle_lds<L>(twl, kp.tw);
 
   const int lf = threadIdx.x / T, t = threadIdx.x % T;
   const int field = blockIdx.x * FPB + lf;
   const bool active = field < kp.nFields;
   cplx* buf = lds + lf * F::lds_elems;
 
-  FftTwiddles<L> tw;
-  fft_load_twiddles<L>(tw, kp.tw, t);
+  typename PropTw<L>::type tw;
+  init_twiddles<L>(tw, kp.tw, twl, t);
   cplx ds[8], ram[8], flt[8];
 #pragma unroll
   for (int m = 0; m < 8; m++) {
@@ -409,7 +423,7 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_gnlse(KParams kp) {
   load_inputs<L, 1>(a, kp, field, t, active);
   prologue<L, 1>(a, kp, buf, t, tw, field, active);
   if (kp.nZ > 1) {
-    fft_field<L, true>(a[0], buf, t, tw);
+    fft_field1<L, true>(a[0], buf, t, tw);
     scale_inv<L>(a[0]);
     if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, 1, 0, 0, N), a[0], t, active);
   }
@@ -441,12 +455,12 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_gnlse(KParams kp) {
 #pragma unroll
     for (int m = 0; m < 8; m++) a[0][m] = a[0][m] + (acc[m] + k[m]) * (1. / 6.);
 
-    fft_field<L, false>(a[0], buf, t, tw);
+    fft_field1<L, false>(a[0], buf, t, tw);
 #pragma unroll
     for (int m = 0; m < 8; m++) a[0][m] = a[0][m] * ds[m];
     if (i < kp.nZ - 1) {
       if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, 1, 0, i, N), a[0], t, active);
-      fft_field<L, true>(a[0], buf, t, tw);
+      fft_field1<L, true>(a[0], buf, t, tw);
       scale_inv<L>(a[0]);
       if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, 1, 0, i, N), a[0], t, active);
     }

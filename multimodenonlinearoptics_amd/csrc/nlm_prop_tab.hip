@@ -78,24 +78,29 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T, FPB = Launch<L>::FPB;
   __shared__ cplx lds[FPB * F::lds_elems];
+  __shared__ cplx twl[PropTw<L>::LDS_ELEMS];
+  fill_twiddle_lds<L>(twl, kp.tw);
+  constexpr bool LEAN = PropTw<L>::LEAN;
 
   const int lf = threadIdx.x / T, t = threadIdx.x % T;
   const int field = blockIdx.x * FPB + lf;
   const bool active = field < kp.nFields;
   cplx* buf = lds + lf * F::lds_elems;
 
-  FftTwiddles<L> tw;
-  fft_load_twiddles<L>(tw, kp.tw, t);
-  cplx ds[8];
+  typename PropTw<L>::type tw;
+  init_twiddles<L>(tw, kp.tw, twl, t);
+  cplx ds[LEAN ? 1 : 8];  // from N = 4096 read from the cached row at each use
+  if constexpr (!LEAN) {
 #pragma unroll
-  for (int m = 0; m < 8; m++) ds[m] = kp.dstep[t + T * m];
+    for (int m = 0; m < 8; m++) ds[m] = kp.dstep[t + T * m];
+  }
 
   cplx a[1][8];
   load_inputs<L, 1>(a, kp, field, t, active);
   prologue<L, 1>(a, kp, buf, t, tw, field, active);
   cplx al[8], be[8];
   if (kp.nZ > 1) {
-    fft_field<L, true>(a[0], buf, t, tw);  // unscaled: the table holds the 1/N
+    fft_field1<L, true>(a[0], buf, t, tw);  // unscaled: the table holds the 1/N
     if (kp.kind == KIND_GRID) store_scaled<L>(grid_row(kp, kp.gridT, field, 1, 0, 0, N), a[0], t, active);
     if (kp.ckpt) store_scaled<L>(kp.ckpt + (size_t)field * kp.nSeg * N, a[0], t, active);
     load_prop_row<L>(al, be, kp.tab + (size_t)N * 2, t);
@@ -112,12 +117,12 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
     if (i + 1 < kp.nZ) load_prop_row<L>(al, be, kp.tab + (size_t)(i + 1) * N * 2, t);
 
     // --- dispersion step (src/_NonlinearMedium.hpp:212-215)
-    fft_field<L, false>(a[0], buf, t, tw);
+    fft_field1<L, false>(a[0], buf, t, tw);
 #pragma unroll
-    for (int m = 0; m < 8; m++) a[0][m] = a[0][m] * ds[m];
+    for (int m = 0; m < 8; m++) a[0][m] = a[0][m] * (LEAN ? kp.dstep[t + T * m] : ds[LEAN ? 0 : m]);
     if (i < kp.nZ - 1) {
       if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, 1, 0, i, N), a[0], t, active);
-      fft_field<L, true>(a[0], buf, t, tw);
+      fft_field1<L, true>(a[0], buf, t, tw);
       if (kp.kind == KIND_GRID) store_scaled<L>(grid_row(kp, kp.gridT, field, 1, 0, i, N), a[0], t, active);
       if (kp.ckpt && i % kp.ckptEvery == 0)
         store_scaled<L>(kp.ckpt + ((size_t)field * kp.nSeg + i / kp.ckptEvery) * N, a[0], t, active);
