@@ -391,7 +391,7 @@ def main():
         print(json.dumps({"copy_probe_bytes": n * 8}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the split-step engine on a single MI355X "
+        print("bench.py measures one GPU: the split-step engine runs on a single MI355X "
               "(no multi-GPU path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 

@@ -1,7 +1,7 @@
 /*
  * nlm_oracle.c -- CPU restatement of the split-step propagation core of
- * `nonlinearmedium`.  TEST INFRASTRUCTURE ONLY (parity checker and CPU
- * baseline); the product never links or loads this file's library.
+ * `nonlinearmedium`.  TEST INFRASTRUCTURE ONLY (the parity reference and the
+ * CPU baseline); the product never links or loads this file's library.
  *
  * Each function names the reference lines it restates.  The arithmetic keeps
  * the reference's operation order (left-to-right evaluation of the Eigen

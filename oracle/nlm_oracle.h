@@ -2,7 +2,7 @@
  * nlm_oracle.h -- CPU restatement of the split-step propagation core of
  * `nonlinearmedium` (mcmahon-lab/MultimodeNonlinearOptics).
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the CPU
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity reference and the CPU
  * baseline timer.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
  * leg of bench.py load it; the product (multimodenonlinearoptics_amd) never
  * links, loads or calls it.

@@ -1,7 +1,7 @@
 """CPU oracle: Python classes over the C restatement in nlm_oracle.c.
 
 TEST INFRASTRUCTURE ONLY.  Used by tests/, by __graft_entry__.smoke() as the
-checker, and by bench.py's cpu_baseline leg.  The product never imports it.
+parity reference, and by bench.py's cpu_baseline leg.  The product never imports it.
 
 Every class mirrors the reference's pybind11 binding of the same name (argument
 names, order and defaults) and maps its arguments onto the base-class
