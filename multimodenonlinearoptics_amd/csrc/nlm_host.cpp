@@ -594,13 +594,13 @@ std::pair<Grid, Grid> Medium::greensForward(bool inTimeDomain, bool runPump, int
 // Chain rule from the step coefficients to the per-step poling values for
 // Chi2PDC (solvers/Chi2PDC.cpp:35-46): c_prev = p[i-1] nl mis(i-1),
 // c_intm = (p[i-1] + p[i]) / 2 nl mis(i-1/2), c_curr = p[i] nl mis(i).
-std::pair<std::vector<cd>, std::vector<double>> Medium::greensAdjoint(const Grid& gC, const Grid& gS) {
+std::pair<std::vector<cd>, std::vector<double>> Medium::greensAdjoint(const cd* gC, const cd* gS, int rows,
+                                                                      int cols) {
   if (model_ != NLM_CHI2PDC) throw std::invalid_argument("the differentiable Green's function is implemented for Chi2PDC");
-  if (gC.rows != N_ || gC.cols != N_ || gS.rows != N_ || gS.cols != N_)
-    throw std::invalid_argument("gradient shapes do not match the Green's function");
+  if (rows != N_ || cols != N_) throw std::invalid_argument("gradient shapes do not match the Green's function");
   nlm_plan* p = plan();
   std::vector<cd> gcoef((size_t)nZ_ * 3), genv(N_);
-  check(nlm_greens_adjoint(p, reinterpret_cast<const double*>(gC.v.data()), reinterpret_cast<const double*>(gS.v.data()),
+  check(nlm_greens_adjoint(p, reinterpret_cast<const double*>(gC), reinterpret_cast<const double*>(gS),
                            reinterpret_cast<double*>(gcoef.data()), nullptr, reinterpret_cast<double*>(genv.data())));
   std::vector<double> gpol(nZ_, 0.0);
   for (unsigned i = 1; i < (unsigned)nZ_; i++) {

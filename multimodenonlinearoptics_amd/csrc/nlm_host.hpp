@@ -101,7 +101,8 @@ class Medium {
   // checkpoints every ckptEvery steps (0: ceil(sqrt(nZ-1))), then the adjoint;
   // returns (dL/d envelope [N] complex, dL/d poling [nZ] real) for dL/dC, dL/dS
   std::pair<Grid, Grid> greensForward(bool inTimeDomain, bool runPump, int ckptEvery);
-  std::pair<std::vector<cd>, std::vector<double>> greensAdjoint(const Grid& gC, const Grid& gS);
+  // gC, gS: rows x cols complex, row-major (dL/dC, dL/dS of the last greensForward)
+  std::pair<std::vector<cd>, std::vector<double>> greensAdjoint(const cd* gC, const cd* gS, int rows, int cols);
   // differentiable batch (Chi3GNLSE): forward keeping checkpoints, then dL/d(inputs)
   Grid batchForward(const Grid& inputProfs, bool inTimeDomain, int ckptEvery);
   // z-resolved batch: runSignalSimulation of every row in one launch; grids of

@@ -245,11 +245,15 @@ PYBIND11_MODULE(nonlinearmedium, m) {
            "inTimeDomain"_a = false, "runPump"_a = true, "ckptEvery"_a = 0);
   base.def("greensAdjoint",
            [](Medium& md, const carr& gC, const carr& gS) {
-             Grid a = to_grid(gC), b = to_grid(gS);
+             // the arrays are contiguous complex128 (converted if needed) and stay
+             // alive for the call: hand their buffers over without a copy
+             if (gC.ndim() != 2 || gS.ndim() != 2 || gC.shape(0) != gS.shape(0) || gC.shape(1) != gS.shape(1))
+               throw std::invalid_argument("gradient shapes do not match the Green's function");
+             const int rows = (int)gC.shape(0), cols = (int)gC.shape(1);
              std::pair<std::vector<cd>, std::vector<double>> r;
              {
                py::gil_scoped_release nogil;
-               r = md.greensAdjoint(a, b);
+               r = md.greensAdjoint(gC.data(), gS.data(), rows, cols);
              }
              py::array_t<std::complex<double>> ge(r.first.size());
              std::copy(r.first.begin(), r.first.end(), ge.mutable_data());
