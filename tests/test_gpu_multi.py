@@ -53,6 +53,33 @@ def test_mixed_batch_frequency_domain_and_sizes(nlm, oracle):
         assert rel_l2(b.output(i), exp) < TOL, i
 
 
+def test_large_pdc_class_matches_oracle(nlm, oracle):
+    """Chi2PDC at N = 4096 (512 lanes per field, pump rows generated in the
+    kernel) with instances of 3, 70 and 150 steps (the generator's phase is
+    re-evaluated exactly every 64 rows) and a Chi2SHG medium of the same size in
+    the same batch."""
+    r = np.random.default_rng(41)
+    kws = [dict(relativeLength=1.0, nlLength=0.5 + 0.3 * j, beta2=0.2 * j - 0.1, beta2s=-0.3, beta1=0.1 * j,
+                diffBeta0=0.5 - j, tMax=20, tPrecision=4096, nZSteps=nz) for j, nz in enumerate((3, 70, 150))]
+    insts = [("Chi2PDC", kw) for kw in kws]
+    insts.append(("Chi2SHG", dict(relativeLength=1.0, nlLengthH=2.0, nlLengthP=1.0, beta2h=-0.2, beta2p=0.1,
+                                  tMax=20, tPrecision=4096, nZSteps=40)))
+    media = [getattr(nlm, n)(**kw) for n, kw in insts]
+    ref = [getattr(oracle, n)(**kw) for n, kw in insts]
+    cols = [kw["tPrecision"] for _, kw in insts]
+    xs = [r.standard_normal(c) + 1j * r.standard_normal(c) for c in cols]
+    b = nlm.MultiBatch(media, xs, inTimeDomain=True, autotune=False)
+    b.run()
+    for i, (rm, x) in enumerate(zip(ref, xs)):
+        print("large-N instance", i, insts[i][0], insts[i][1]["nZSteps"], flush=True)
+        exp = rm.batchSignalSimulation(x[None, :], inTimeDomain=True)[0]
+        assert rel_l2(b.output(i), exp) < TOL, (i, rel_l2(b.output(i), exp))
+    first = [b.output(i) for i in range(len(b))]
+    b.run()
+    for i in range(len(b)):
+        assert np.array_equal(first[i], b.output(i))
+
+
 def test_mixed_batch_errors(nlm):
     chi3 = nlm.Chi3(relativeLength=1, nlLength=1, beta2=-1, tMax=10, tPrecision=64, nZSteps=10)
     with pytest.raises(ValueError, match="heterogeneous batches support"):
