@@ -183,9 +183,15 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
       ap.iLo = iHi - chunk > 0 ? iHi - chunk : 0;
       sweep(ap, nWG, p->stream);
       const int count = ap.iHi - ap.iLo;
-      const unsigned blocks = (unsigned)(count * ((N + 63) / 64));
-      hipLaunchKernelGGL(k_adj_reduce, dim3(blocks), dim3(64 * RED_WAVES), 0, p->stream, ap.part, nWG, chunk, ap.iLo,
-                         N, p->coef.p, p->pumpT[0].p, p->gcoefPart.p);
+      if (N >= 256) {  // four points per lane: 1 KB per load instruction, 4 KB per wave and partial
+        const unsigned blocks = (unsigned)(count * (N / 256));
+        hipLaunchKernelGGL(k_adj_reduce<4>, dim3(blocks), dim3(64 * RED_WAVES), 0, p->stream, ap.part, nWG, chunk,
+                           ap.iLo, N, p->coef.p, p->pumpT[0].p, p->gcoefPart.p);
+      } else {
+        const unsigned blocks = (unsigned)(count * ((N + 63) / 64));
+        hipLaunchKernelGGL(k_adj_reduce<1>, dim3(blocks), dim3(64 * RED_WAVES), 0, p->stream, ap.part, nWG, chunk,
+                           ap.iLo, N, p->coef.p, p->pumpT[0].p, p->gcoefPart.p);
+      }
     }
   }
   HIP_TRY(hipGetLastError());

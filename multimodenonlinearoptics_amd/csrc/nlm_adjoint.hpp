@@ -302,89 +302,106 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
 // (alpha, beta) to m_s = c_s P_s: R(x) = x + (k1 + 2 k2 + 2 k3 + k4) / 6 with
 // k_s = m_s conj(y_s) (Chi2PDC::DiffEq, solvers/Chi2PDC.cpp:27-47),
 // alpha = (R(1) - i R(i)) / 2, beta = (R(1) + i R(i)) / 2.  Writes gm[i][s][tau].
-// One block per (step, 64 time points): wave q sums workgroups
-// q, q + RED_WAVES, ... (fixed order), then lane tau adds the wave sums in order.
+// One block per (step, 64 TPL time points): wave q sums workgroups
+// q, q + RED_WAVES, ... (fixed order), lane l taking the points k0 + l + 64 u
+// (u < TPL, so each load instruction of a wave reads 1 KB contiguous and a wave
+// streams 64 TPL points of each partial); then wave 0 adds the wave sums in order.
 constexpr int RED_WAVES = 8;
 constexpr int RED_ACC = 4;
+template <int TPL>
 __global__ void __launch_bounds__(64 * RED_WAVES) k_adj_reduce(const cplx* __restrict__ part, int nWG, int chunk,
                                                                int iLo, int N, const cplx* __restrict__ coef,
                                                                const cplx* __restrict__ pump, cplx* __restrict__ gm) {
-  __shared__ cplx red[2][RED_WAVES][64];
-  const int tiles = (N + 63) / 64;
-  const int jj = blockIdx.x / tiles, k0 = (blockIdx.x % tiles) * 64;
-  const int q = threadIdx.x / 64, lane = threadIdx.x % 64, k = k0 + lane;
-  cplx ga = mk(0, 0), gb = mk(0, 0);
-  if (k < N) {
-    // RED_ACC independent accumulators (workgroups w + RED_WAVES u of each group
-    // of RED_ACC RED_WAVES) keep that many loads in flight per lane; the
+  constexpr int TILE = 64 * TPL;
+  __shared__ cplx red[2][RED_WAVES][TILE];
+  const int tiles = (N + TILE - 1) / TILE;
+  const int jj = blockIdx.x / tiles, k0 = (blockIdx.x % tiles) * TILE;
+  const int q = threadIdx.x / 64, lane = threadIdx.x % 64;
+  // TPL > 1 is launched only when TILE divides N
+  const bool valid = TPL > 1 || k0 + lane < N;
+  {
+    // RED_ACC independent accumulators (workgroups w + RED_WAVES v of each group
+    // of RED_ACC RED_WAVES) keep that many loads in flight per point; the
     // summation order is fixed
-    cplx a4[RED_ACC], b4[RED_ACC];
+    cplx a4[TPL][RED_ACC], b4[TPL][RED_ACC];
 #pragma unroll
-    for (int u = 0; u < RED_ACC; u++) a4[u] = b4[u] = ga;
-    const size_t stride = (size_t)RED_WAVES * chunk * 2 * N;
-    const cplx* p = part + (((size_t)q * chunk + jj) * 2) * N + k;
-    int w = q;
-    for (; w + (RED_ACC - 1) * RED_WAVES < nWG; w += RED_ACC * RED_WAVES, p += RED_ACC * stride) {
+    for (int u = 0; u < TPL; u++)
 #pragma unroll
-      for (int u = 0; u < RED_ACC; u++) {
-        a4[u] = a4[u] + load_stream(p + u * stride);
-        b4[u] = b4[u] + load_stream(p + u * stride + N);
+      for (int v = 0; v < RED_ACC; v++) a4[u][v] = b4[u][v] = mk(0, 0);
+    if (valid) {
+      const size_t stride = (size_t)RED_WAVES * chunk * 2 * N;
+      const cplx* p = part + (((size_t)q * chunk + jj) * 2) * N + k0 + lane;
+      int w = q;
+      for (; w + (RED_ACC - 1) * RED_WAVES < nWG; w += RED_ACC * RED_WAVES, p += RED_ACC * stride) {
+#pragma unroll
+        for (int v = 0; v < RED_ACC; v++)
+#pragma unroll
+          for (int u = 0; u < TPL; u++) {
+            a4[u][v] = a4[u][v] + load_stream(p + v * stride + 64 * u);
+            b4[u][v] = b4[u][v] + load_stream(p + v * stride + N + 64 * u);
+          }
       }
-    }
-    for (int u = 0; w < nWG; w += RED_WAVES, p += stride, u++) {
-      a4[u] = a4[u] + load_stream(p);
-      b4[u] = b4[u] + load_stream(p + N);
+      for (int v = 0; w < nWG; w += RED_WAVES, p += stride, v++)
+#pragma unroll
+        for (int u = 0; u < TPL; u++) {
+          a4[u][v] = a4[u][v] + load_stream(p + 64 * u);
+          b4[u][v] = b4[u][v] + load_stream(p + N + 64 * u);
+        }
     }
 #pragma unroll
-    for (int h = RED_ACC / 2; h >= 1; h /= 2) {
+    for (int u = 0; u < TPL; u++) {
 #pragma unroll
-      for (int u = 0; u < h; u++) {
-        a4[u] = a4[u] + a4[u + h];
-        b4[u] = b4[u] + b4[u + h];
+      for (int h = RED_ACC / 2; h >= 1; h /= 2) {
+#pragma unroll
+        for (int v = 0; v < h; v++) {
+          a4[u][v] = a4[u][v] + a4[u][v + h];
+          b4[u][v] = b4[u][v] + b4[u][v + h];
+        }
       }
+      red[0][q][lane + 64 * u] = a4[u][0];
+      red[1][q][lane + 64 * u] = b4[u][0];
     }
-    ga = a4[0];
-    gb = b4[0];
   }
-  red[0][q][lane] = ga;
-  red[1][q][lane] = gb;
   __syncthreads();
-  if (q != 0 || k >= N) return;
-  ga = red[0][0][lane];
-  gb = red[1][0][lane];
+  if (q != 0 || !valid) return;
+#pragma unroll 1
+  for (int u = 0; u < TPL; u++) {
+    const int k = k0 + lane + 64 * u;
+    cplx ga = red[0][0][lane + 64 * u], gb = red[1][0][lane + 64 * u];
 #pragma unroll
-  for (int r = 1; r < RED_WAVES; r++) {
-    ga = ga + red[0][r][lane];
-    gb = gb + red[1][r][lane];
-  }
-  const int i = iLo + 1 + jj;
-  const cplx* c = coef + (size_t)i * 3;
-  const cplx m0 = c[0] * pump[(size_t)(2 * i - 2) * N + k];
-  const cplx m1 = c[1] * pump[(size_t)(2 * i - 1) * N + k];
-  const cplx m2 = c[2] * pump[(size_t)(2 * i) * N + k];
-  cplx gm0 = mk(0, 0), gm1 = mk(0, 0), gm2 = mk(0, 0);
+    for (int r = 1; r < RED_WAVES; r++) {
+      ga = ga + red[0][r][lane + 64 * u];
+      gb = gb + red[1][r][lane + 64 * u];
+    }
+    const int i = iLo + 1 + jj;
+    const cplx* c = coef + (size_t)i * 3;
+    const cplx m0 = c[0] * pump[(size_t)(2 * i - 2) * N + k];
+    const cplx m1 = c[1] * pump[(size_t)(2 * i - 1) * N + k];
+    const cplx m2 = c[2] * pump[(size_t)(2 * i) * N + k];
+    cplx gm0 = mk(0, 0), gm1 = mk(0, 0), gm2 = mk(0, 0);
 #pragma unroll
-  for (int pass = 0; pass < 2; pass++) {
-    // x = 1 carries g_R(1) = (g_alpha + g_beta) / 2, x = i carries g_R(i) = i (g_alpha - g_beta) / 2
-    const cplx x = pass == 0 ? mk(1, 0) : mk(0, 1);
-    const cplx gr = pass == 0 ? 0.5 * (ga + gb) : mk(0, 0.5) * (ga - gb);
-    const cplx y2 = x + 0.5 * (m0 * conj(x));
-    const cplx y3 = x + 0.5 * (m1 * conj(y2));
-    const cplx y4 = x + m1 * conj(y3);
-    const cplx gk4 = gr * (1. / 6.);
-    cplx gk3 = gr * (1. / 3.), gk2 = gr * (1. / 3.), gk1 = gk4;
-    gm2 = gm2 + y4 * gk4;
-    gk3 = gk3 + m2 * conj(gk4);
-    gm1 = gm1 + y3 * gk3;
-    gk2 = gk2 + 0.5 * (m1 * conj(gk3));
-    gm1 = gm1 + y2 * gk2;
-    gk1 = gk1 + 0.5 * (m1 * conj(gk2));
-    gm0 = gm0 + x * gk1;
+    for (int pass = 0; pass < 2; pass++) {
+      // x = 1 carries g_R(1) = (g_alpha + g_beta) / 2, x = i carries g_R(i) = i (g_alpha - g_beta) / 2
+      const cplx x = pass == 0 ? mk(1, 0) : mk(0, 1);
+      const cplx gr = pass == 0 ? 0.5 * (ga + gb) : mk(0, 0.5) * (ga - gb);
+      const cplx y2 = x + 0.5 * (m0 * conj(x));
+      const cplx y3 = x + 0.5 * (m1 * conj(y2));
+      const cplx y4 = x + m1 * conj(y3);
+      const cplx gk4 = gr * (1. / 6.);
+      cplx gk3 = gr * (1. / 3.), gk2 = gr * (1. / 3.), gk1 = gk4;
+      gm2 = gm2 + y4 * gk4;
+      gk3 = gk3 + m2 * conj(gk4);
+      gm1 = gm1 + y3 * gk3;
+      gk2 = gk2 + 0.5 * (m1 * conj(gk3));
+      gm1 = gm1 + y2 * gk2;
+      gk1 = gk1 + 0.5 * (m1 * conj(gk2));
+      gm0 = gm0 + x * gk1;
+    }
+    cplx* o = gm + (size_t)i * 3 * N;
+    o[k] = gm0;
+    o[N + k] = gm1;
+    o[2 * N + k] = gm2;
   }
-  cplx* o = gm + (size_t)i * 3 * N;
-  o[k] = gm0;
-  o[N + k] = gm1;
-  o[2 * N + k] = gm2;
 }
 
 // g_coef[i][s] = sum_tau conj(P_s) gm[i][s] (fixed-order tree; one block per step).
