@@ -121,6 +121,17 @@ def test_adjoint_errors(nlm):
     m.greensForward(False, True, 0)
     with pytest.raises(ValueError):
         m.greensAdjoint(np.zeros((N, N + 1), complex), np.zeros((N, N), complex))
+    with pytest.raises(ValueError):
+        m.greensAdjoint(np.zeros((N, N + 1), complex), np.zeros((N, N + 1), complex))
+    with pytest.raises(ValueError):
+        m.greensAdjoint(np.zeros(N * N, complex), np.zeros(N * N, complex))
+    # non-contiguous and real inputs are converted (the buffers are otherwise used in place)
+    r = np.random.default_rng(9)
+    gC = r.standard_normal((N, 2 * N))[:, ::2] + 0j
+    gS = r.standard_normal((N, N))
+    a = m.greensAdjoint(gC, gS)
+    b = m.greensAdjoint(np.ascontiguousarray(gC), gS.astype(complex))
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     sfg = nlm.Chi2SFG(relativeLength=1, nlLength=1, nlLengthOrig=1, beta2=0.1, beta2s=0.1, beta2o=0.1, tMax=10,
                       tPrecision=64)
     with pytest.raises(ValueError):
