@@ -138,11 +138,12 @@ def test_adjoint_errors(nlm):
         sfg.greensForward(False, True, 0)
 
 
-@pytest.mark.parametrize("N,nZ,ckpt", [(1024, 33, 0), (2048, 17, 4), (256, 50, 7)])
+@pytest.mark.parametrize("N,nZ,ckpt", [(1024, 33, 0), (2048, 17, 4), (256, 50, 7), (4096, 9, 3), (8192, 5, 2)])
 def test_gradients_match_fd_of_device_forward(nlm, N, nZ, ckpt):
-    """Larger sizes (one and two fields per workgroup in the reverse sweep): the
-    finite differences come from the device forward, itself pinned to the oracle
-    by the parity tests."""
+    """Larger sizes (one and two fields per workgroup in the reverse sweep, and
+    512 / 1024 lanes per field from N = 4096): the finite differences come from
+    the device forward, itself pinned to the oracle by the parity tests."""
+    print("adjoint vs device FD, N=%d nZ=%d ckpt=%d" % (N, nZ, ckpt), flush=True)
     kw, env, pol = _kwargs(N, nZ, SEED0 + 7 * N + nZ)
     r = np.random.default_rng(13)
     Wc = r.standard_normal((N, N)) + 1j * r.standard_normal((N, N))
