@@ -158,15 +158,21 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T, E = F::lds_elems;
   constexpr bool FUSE = AdjShape<L>::FUSE;
+  // From N = 4096 a field's 512 / 1024 lanes have 256 / 128 registers each: the
+  // twiddles come from the table (PropTw), the dispersion row and the
+  // propagator rows are read where they are used instead of being held.
+  constexpr bool LEAN = PropTw<L>::LEAN;
   __shared__ cplx lds[AdjShape<L>::LDS_ELEMS];
+  __shared__ cplx twl[PropTw<L>::LDS_ELEMS];
+  fill_twiddle_lds<L>(twl, ap.tw);
   const int lf = threadIdx.x / T, t = threadIdx.x % T;
   const int field = blockIdx.x * FA + lf;
   const bool active = field < ap.nFields;
   const int fld = active ? field : 0;
   cplx* buf = lds + lf * E;
 
-  RegTwiddles<L> tw;
-  fft_load_twiddles<L>(tw.w, ap.tw, t);
+  typename PropTw<L>::type tw;
+  init_twiddles<L>(tw, ap.tw, twl, t);
   cplx g[8], x[8];
   bool gFreq;
   if (ap.iHi == ap.nZ - 1) {
@@ -209,15 +215,20 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
     gFreq = false;
   }
 
-  cplx ds[8];  // conj(exp(i dz D)) / N
+  cplx dsr[LEAN ? 1 : 8];  // conj(exp(i dz D)) / N
+  if constexpr (!LEAN) {
 #pragma unroll
-  for (int m = 0; m < 8; m++) ds[m] = conj(ap.dstep[t + T * m]) * (1.0 / N);
+    for (int m = 0; m < 8; m++) dsr[m] = conj(ap.dstep[t + T * m]) * (1.0 / N);
+  }
   // propagator rows (alpha, beta) / N of this step (al, be) and of the step
   // before it (pa, pb, which also inverts it), loaded a step ahead so their
-  // latency hides behind the transforms
-  cplx al[8], be[8], pa[8], pb[8];
-  prop_row<L>(al, be, ap.tab + (size_t)ap.iHi * N * 2, t);
-  if (ap.iHi >= 2) prop_row<L>(pa, pb, ap.tab + (size_t)(ap.iHi - 1) * N * 2, t);
+  // latency hides behind the transforms (at the point of use from N = 4096)
+  constexpr int PR = LEAN ? 1 : 8;
+  cplx al[8], be[8], pa[PR], pb[PR];
+  if constexpr (!LEAN) {
+    prop_row<L>(al, be, ap.tab + (size_t)ap.iHi * N * 2, t);
+    if (ap.iHi >= 2) prop_row<L>(pa, pb, ap.tab + (size_t)(ap.iHi - 1) * N * 2, t);
+  }
 
   for (int i = ap.iHi; i > ap.iLo; i--) {
     // adjoint of the dispersion step: h = g_r = IFFT(conj(exp(i dz D)) FFT(g));
@@ -232,8 +243,8 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
       fft_field2<L, false>(g, xr, buf, lds + (FA + lf) * E, t, tw);
 #pragma unroll
       for (int m = 0; m < 8; m++) {
-        g[m] = ds[m] * g[m];
-        xr[m] = ds[m] * xr[m];
+        g[m] = (LEAN ? conj(ap.dstep[t + T * m]) * (1.0 / N) : dsr[LEAN ? 0 : m]) * g[m];
+        xr[m] = (LEAN ? conj(ap.dstep[t + T * m]) * (1.0 / N) : dsr[LEAN ? 0 : m]) * xr[m];
       }
       fft_field2<L, true>(g, xr, buf, lds + (FA + lf) * E, t, tw);
     } else {
@@ -244,7 +255,7 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
         for (int m = 0; m < 8; m++) g[m] = g[m] * (double)N;  // ds carries 1/N
       }
 #pragma unroll
-      for (int m = 0; m < 8; m++) g[m] = ds[m] * g[m];
+      for (int m = 0; m < 8; m++) g[m] = (LEAN ? conj(ap.dstep[t + T * m]) * (1.0 / N) : dsr[LEAN ? 0 : m]) * g[m];
       fft_field1<L, true>(g, buf, t, tw);  // unscaled inverse
     }
     gFreq = false;
@@ -258,6 +269,7 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
       wg_sum_product<L, FA, true>(x, g, active, lds, lf, t, dst);
       wg_sum_product<L, FA, false>(x, g, active, lds, lf, t, dst + N);
     }
+    if constexpr (LEAN) prop_row<L>(al, be, ap.tab + (size_t)i * N * 2, t);
 #pragma unroll
     for (int m = 0; m < 8; m++) {
       const cplx h = g[m], a = al[m] * (double)N, b = be[m] * (double)N;
@@ -275,17 +287,24 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
           for (int m = 0; m < 8; m++) xr[m] = x[m];
           fft_field1<L, false>(xr, buf, t, tw);
 #pragma unroll
-          for (int m = 0; m < 8; m++) xr[m] = ds[m] * xr[m];
+          for (int m = 0; m < 8; m++) xr[m] = (LEAN ? conj(ap.dstep[t + T * m]) * (1.0 / N) : dsr[LEAN ? 0 : m]) * xr[m];
           fft_field1<L, true>(xr, buf, t, tw);
         }
-        inverse_step<L>(x, xr, pa, pb);
+        if constexpr (LEAN) {
+          prop_row<L>(al, be, ap.tab + (size_t)(i - 1) * N * 2, t);
+          inverse_step<L>(x, xr, al, be);
+        } else {
+          inverse_step<L>(x, xr, pa, pb);
+        }
       }
+      if constexpr (!LEAN) {
 #pragma unroll
-      for (int m = 0; m < 8; m++) {
-        al[m] = pa[m];
-        be[m] = pb[m];
+        for (int m = 0; m < 8; m++) {
+          al[m] = pa[m];
+          be[m] = pb[m];
+        }
+        if (i >= 3) prop_row<L>(pa, pb, ap.tab + (size_t)(i - 2) * N * 2, t);
       }
-      if (i >= 3) prop_row<L>(pa, pb, ap.tab + (size_t)(i - 2) * N * 2, t);
     }
   }
   if (ap.iLo > 0 && active) {
