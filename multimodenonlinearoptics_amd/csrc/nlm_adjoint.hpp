@@ -60,9 +60,18 @@ struct AdjShape {
   static constexpr int byLanes = T < 256 ? 256 / T : 1;
   static constexpr int m1 = byLds < byWG ? byLds : byWG;
   static constexpr int m2 = m1 < byLanes ? m1 : byLanes;
-  static constexpr int FA = m2 >= 4 ? 4 : m2 >= 2 ? 2 : 1;
+  // N = 2048: two fields per workgroup (512 lanes, two waves per SIMD, lean
+  // registers below), which halves the partials against one field per workgroup
+  static constexpr int FA = L == 11 ? 2 : m2 >= 4 ? 4 : m2 >= 2 ? 2 : 1;
   static constexpr int THREADS = FA * T;
-  static constexpr bool FUSE = 2 * FA * FftShape<L>::lds_elems * 16 <= 160 * 1024 && THREADS <= 256;
+  // LEAN: fewer than 512 registers per lane (two or more waves per SIMD): the
+  // twiddles come from the table (an LDS copy up to N = 4096, the global table
+  // at N = 8192) and the dispersion and propagator rows are read where used
+  static constexpr bool LEAN = L >= 12 || THREADS > 256;
+  static constexpr bool TW_LDS = LEAN && L <= 12;
+  static constexpr int TW_ELEMS = TW_LDS ? N : 1;
+  static constexpr bool FUSE =
+      !LEAN && 2 * FA * FftShape<L>::lds_elems * 16 <= 160 * 1024 && THREADS <= 256;
   static constexpr int BUFS = FUSE ? 2 * FA : FA;  // LDS transform buffers per workgroup
   static constexpr int LDS_ELEMS = BUFS * FftShape<L>::lds_elems;
 };
@@ -158,21 +167,29 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T, E = F::lds_elems;
   constexpr bool FUSE = AdjShape<L>::FUSE;
-  // From N = 4096 a field's 512 / 1024 lanes have 256 / 128 registers each: the
-  // twiddles come from the table (PropTw), the dispersion row and the
-  // propagator rows are read where they are used instead of being held.
-  constexpr bool LEAN = PropTw<L>::LEAN;
+  // With 256 or 128 registers per lane (AdjShape::LEAN) the twiddles come from
+  // the table and the dispersion and propagator rows are read where they are
+  // used instead of being held.
+  constexpr bool LEAN = AdjShape<L>::LEAN;
   __shared__ cplx lds[AdjShape<L>::LDS_ELEMS];
-  __shared__ cplx twl[PropTw<L>::LDS_ELEMS];
-  fill_twiddle_lds<L>(twl, ap.tw);
+  __shared__ cplx twl[AdjShape<L>::TW_ELEMS];
+  if constexpr (AdjShape<L>::TW_LDS) {
+    for (int k = threadIdx.x; k < N; k += blockDim.x) twl[k] = ap.tw[k];
+    __syncthreads();
+  }
   const int lf = threadIdx.x / T, t = threadIdx.x % T;
   const int field = blockIdx.x * FA + lf;
   const bool active = field < ap.nFields;
   const int fld = active ? field : 0;
   cplx* buf = lds + lf * E;
 
-  typename PropTw<L>::type tw;
-  init_twiddles<L>(tw, ap.tw, twl, t);
+  typename std::conditional<LEAN, TableTwiddles<L>, RegTwiddles<L>>::type tw;
+  if constexpr (LEAN) {
+    tw.tab = AdjShape<L>::TW_LDS ? twl : ap.tw;
+    tw.t = t;
+  } else {
+    fft_load_twiddles<L>(tw.w, ap.tw, t);
+  }
   cplx g[8], x[8];
   bool gFreq;
   if (ap.iHi == ap.nZ - 1) {
