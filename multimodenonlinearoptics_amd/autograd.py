@@ -20,6 +20,14 @@ import numpy as np
 import torch
 
 
+def _check_serial(ctx):
+    # the medium keeps the checkpoints of its latest forward only
+    if ctx.medium.forwardSerial != ctx.serial:
+        raise RuntimeError("the medium ran another differentiable forward after this one; its checkpoints are "
+                           "gone: run backward before the next forward on the same medium, or use one medium "
+                           "per graph")
+
+
 class GreensFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, medium, envelope, poling, ckpt_every=0):
@@ -29,11 +37,13 @@ class GreensFunction(torch.autograd.Function):
         medium.setPolingProfile(pol)
         C, S = medium.greensForward(False, True, int(ckpt_every))
         ctx.medium = medium
+        ctx.serial = medium.forwardSerial
         ctx.device = envelope.device
         return torch.from_numpy(C).to(envelope.device), torch.from_numpy(S).to(envelope.device)
 
     @staticmethod
     def backward(ctx, gC, gS):
+        _check_serial(ctx)
         gC = np.zeros(0) if gC is None else gC.detach().cpu().numpy()
         gS = np.zeros(0) if gS is None else gS.detach().cpu().numpy()
         n = ctx.medium.signalFreq.shape[1]
@@ -51,11 +61,13 @@ class BatchSignal(torch.autograd.Function):
         x = inputs.detach().cpu().numpy().astype(np.complex128)
         out = medium.batchForward(x, bool(inTimeDomain), int(ckpt_every))
         ctx.medium = medium
+        ctx.serial = medium.forwardSerial
         ctx.device = inputs.device
         return torch.from_numpy(out).to(inputs.device)
 
     @staticmethod
     def backward(ctx, g):
+        _check_serial(ctx)
         gin = ctx.medium.batchAdjoint(g.detach().cpu().numpy().astype(np.complex128))
         return None, torch.from_numpy(gin).to(ctx.device), None, None
 

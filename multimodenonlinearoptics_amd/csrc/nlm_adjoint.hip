@@ -114,6 +114,7 @@ extern "C" int nlm_greens_forward(nlm_plan* p, int in_time, int n_in, const int3
   p->gNIn = 1;
   p->gNOut = 1;
   p->gInTime = in_time;
+  p->adjGen = ++p->gen;
   (void)block_scale;
   if (C) HIP_TRY(hipMemcpyAsync(C, p->C.p, total * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
   if (S) HIP_TRY(hipMemcpyAsync(S, p->S.p, total * sizeof(cplx), hipMemcpyDeviceToHost, p->stream));
@@ -129,15 +130,22 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
                                   double* g_pump_time, double* g_envelope) {
   if (!p || !gC || !gS || !g_coef) return set_err(NLM_INVALID_ARGUMENT, "null argument");
   if (!p->nSeg || !p->ckpt.p) return set_err(NLM_RUNTIME_ERROR, "run nlm_greens_forward before the adjoint");
+  if (p->adjGen != p->gen)
+    return set_err(NLM_RUNTIME_ERROR,
+                   "the medium changed after nlm_greens_forward (coefficients, pump or another Green's run); run the "
+                   "forward again before the adjoint");
   if (g_envelope && !p->pumpLinear)
     return set_err(NLM_INVALID_ARGUMENT, "envelope gradient needs the linear pump simulation");
   const int N = p->N, L = p->L, nZ = p->nZ, nZP = 2 * nZ - 1;
   const size_t nF = 2 * (size_t)N, W = N, total = W * N;
   const int FA = sweep_fields_per_wg(L);
   const int nWG = (int)((nF + FA - 1) / FA);
-  // steps per sweep launch: the workgroup partials of one launch stay within ~2 GiB
-  // (fewer, longer launches: 1 GiB chunks cost 0.3 % more, 0.5 GiB chunks 10 % more
-  // at N = 1024, nZ = 2000)
+  // steps per sweep launch: the workgroup partials of one launch are kept within
+  // 2 GiB where one step's partials fit (fewer, longer launches: 1 GiB chunks cost
+  // 0.3 % more, 0.5 GiB chunks 10 % more at N = 1024, nZ = 2000).  One step's
+  // partials are nWG * 2N * 16 B: 128 MiB at N = 2048 (16 steps per launch), 1 GiB
+  // at N = 4096, 4 GiB at N = 8192, where a launch holds one step and the buffer
+  // is 4 GiB
   const size_t perStep = (size_t)nWG * 2 * N;
   int chunk = (int)std::max<size_t>(1, ((size_t)1 << 31) / (perStep * sizeof(cplx)));
   if (chunk > nZ - 1) chunk = nZ - 1 > 0 ? nZ - 1 : 1;

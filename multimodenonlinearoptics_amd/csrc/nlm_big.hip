@@ -611,6 +611,7 @@ extern "C" int nlm_batch_forward(nlm_plan* p, const double* inputs, int rows, in
   p->nSeg = nSeg;
   p->gInTime = in_time;
   p->bRows = rows;
+  p->adjGen = ++p->gen;
   return NLM_OK;
 }
 
@@ -619,6 +620,10 @@ extern "C" int nlm_batch_adjoint(nlm_plan* p, const double* g_out, double* g_in)
   if (!g_out || !g_in) return set_err(NLM_INVALID_ARGUMENT, "null argument");
   if (p->bRows <= 0 || !p->nSeg || !p->ckpt.p)
     return set_err(NLM_RUNTIME_ERROR, "run nlm_batch_forward before the adjoint");
+  if (p->adjGen != p->gen)
+    return set_err(NLM_RUNTIME_ERROR,
+                   "the medium changed after nlm_batch_forward (coefficients or another run); run the forward again "
+                   "before the adjoint");
   const int N = p->N, nSteps = p->nZ - 1, rows = p->bRows, K = p->ckptEvery, nSeg = p->nSeg;
   const size_t BN = (size_t)rows * N;
   const Stager sg = pick_stager(p->L);

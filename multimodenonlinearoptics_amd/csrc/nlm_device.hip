@@ -520,6 +520,7 @@ extern "C" int nlm_plan_sync(nlm_plan* p) {
 
 extern "C" int nlm_plan_set_coef(nlm_plan* p, const double* coef) {
   if (!p || !coef) return set_err(NLM_INVALID_ARGUMENT, "null argument");
+  plan_touch(p);
   return upload(p->coef, coef, (size_t)p->nZ * 3 * (p->NC > 0 ? p->NC : 1), p->stream);
 }
 
@@ -555,6 +556,7 @@ extern "C" int nlm_pump_linear(nlm_plan* p, int n_pump, const double* envelope, 
                                const double* strength) {
   if (!p || !envelope || !disp_pump) return set_err(NLM_INVALID_ARGUMENT, "null argument");
   if (n_pump != p->nP || p->nP == 0) return set_err(NLM_INVALID_ARGUMENT, "pump count does not match the model");
+  plan_touch(p);
   if (int e = ensure_pumps(p)) return e;
   const int N = p->N, nZP = 2 * p->nZ - 1;
   if (int e = upload(p->env, envelope, (size_t)p->nP * N, p->stream)) return e;
@@ -584,6 +586,7 @@ extern "C" int nlm_pump_chi3(nlm_plan* p, const double* envelope, const double* 
   if (!p || !envelope || !disp_pump || !disp_step_pump || !rel_intensity)
     return set_err(NLM_INVALID_ARGUMENT, "null argument");
   if (p->nP != 1) return set_err(NLM_INVALID_ARGUMENT, "model has no single pump");
+  plan_touch(p);
   if (int e = ensure_pumps(p)) return e;
   const int N = p->N, nZP = 2 * p->nZ - 1;
   if (int e = upload(p->env, envelope, N, p->stream)) return e;
@@ -604,6 +607,7 @@ extern "C" int nlm_pump_chi3(nlm_plan* p, const double* envelope, const double* 
 extern "C" int nlm_pump_upload(nlm_plan* p, int q, const double* pump_time, const double* pump_freq) {
   if (!p) return set_err(NLM_INVALID_ARGUMENT, "null plan");
   if (q < 0 || q >= p->nP) return set_err(NLM_INVALID_ARGUMENT, "Invalid pump index");
+  plan_touch(p);
   if (int e = ensure_pumps(p)) return e;
   const size_t cnt = (2 * (size_t)p->nZ - 1) * p->N;
   if (pump_time) HIP_TRY(hipMemcpyAsync(p->pumpT[q].p, pump_time, cnt * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
@@ -635,6 +639,7 @@ extern "C" int nlm_pump_import(nlm_plan* p, int q, nlm_plan* src, int mode, cons
   if (src->nZ < p->nZ) return set_err(NLM_INVALID_ARGUMENT, "Medium does not have sufficient resolution to be used with this one");
   if (!src->gridF.p) return set_err(NLM_RUNTIME_ERROR, "no signal grid has been computed");
   const int N = p->N, nZP = 2 * p->nZ - 1;
+  plan_touch(p);
   if (int e = ensure_pumps(p)) return e;
   DevBuf<double> tab;
   if (int e = tab.ensure(3 * (size_t)N)) return e;
@@ -871,6 +876,7 @@ extern "C" int nlm_greens(nlm_plan* p, int in_time, int n_in, const int32_t* inp
   if (n_in < 1 || n_in > p->nS || n_out < 1 || n_out > p->nS)
     return set_err(NLM_INVALID_ARGUMENT, "invalid number of input / output modes");
   if (int e = check_pumps_ready(p)) return e;
+  plan_touch(p);  // G is overwritten
   const int N = p->N;
   const size_t W = (size_t)n_out * N, nF = 2 * (size_t)n_in * N;
   if (int e = p->G.ensure(nF * W)) return e;

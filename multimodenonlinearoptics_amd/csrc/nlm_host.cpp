@@ -588,6 +588,7 @@ std::pair<Grid, Grid> Medium::greensForward(bool inTimeDomain, bool runPump, int
   const int32_t m0 = 0;
   check(nlm_greens_forward(p, inTimeDomain ? 1 : 0, 1, &m0, 1, &m0, nullptr, ckptEvery,
                            reinterpret_cast<double*>(res.first.v.data()), reinterpret_cast<double*>(res.second.v.data())));
+  ++forwardSerial_;
   return res;
 }
 
@@ -625,6 +626,7 @@ Grid Medium::batchForward(const Grid& in, bool inTimeDomain, int ckptEvery) {
   check(nlm_batch_forward(plan(), reinterpret_cast<const double*>(in.v.data()), in.rows, inTimeDomain ? 1 : 0,
                           ckptEvery, reinterpret_cast<double*>(out.v.data())));
   batchRows_ = in.rows;
+  ++forwardSerial_;
   return out;
 }
 

@@ -113,6 +113,7 @@ class Medium {
   void setStepTable(bool on);
   std::vector<double> lastTiming();
   void sync();
+  unsigned long long forwardSerial() const { return forwardSerial_; }
 
   const std::vector<cd>& raman() const { return raman_; }
 
@@ -188,6 +189,9 @@ class Medium {
   bool signalOnDevice_ = false;  // the plan holds the signal grids of the last runSignalSimulation
   bool pumpComputed_ = false;
   int batchRows_ = 0;  // rows of the last batchForward
+  // number of differentiable forwards run so far (greensForward, batchForward): an
+  // autograd caller records it after its forward and checks it before the adjoint
+  unsigned long long forwardSerial_ = 0;
   int lastGreensRows_ = 0, lastGreensCols_ = 0;  // shape of the last Green's function
 };
 

@@ -79,7 +79,13 @@ struct nlm_plan_s {
   // differentiable Chi3GNLSE batch (nlm_batch_forward / nlm_batch_adjoint)
   int bRows = 0;
   DevBuf<cplx> adjWork, adjSeg;
+  // state generation: bumped by every change the adjoints depend on (coefficients,
+  // pump grids, the Green's rows in G); adjGen is the generation a differentiable
+  // forward left, which its adjoint requires unchanged
+  unsigned long long gen = 0, adjGen = 0;
 };
+
+inline void plan_touch(nlm_plan_s* p) { ++p->gen; }
 
 void fill_common(nlm_plan_s* p, KParams& kp);
 int check_pumps_ready(nlm_plan_s* p);

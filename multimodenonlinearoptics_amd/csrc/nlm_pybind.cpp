@@ -303,6 +303,9 @@ PYBIND11_MODULE(nonlinearmedium, m) {
            },
            "dL/d(inputProfs) of the last batchForward for dL/d(outputs) (convention dL/dRe + i dL/dIm; "
            "extension; Chi3GNLSE).", "gOut"_a);
+  base.def_property_readonly("forwardSerial", &Medium::forwardSerial,
+                             "Number of greensForward / batchForward calls made on this medium (extension): "
+                             "an adjoint belongs to the forward whose serial is current.");
   base.def("enableTiming", &Medium::enableTiming, "Record HIP-event kernel times of each run (extension).", "on"_a);
   base.def("setStepTable", &Medium::setStepTable,
            "One-mode undepleted media: propagate with precomputed RK4 step propagators (default) or evaluate "

@@ -154,11 +154,11 @@ def config2(mod, N=512, nZ=1000):
     return mod.Chi2PDC(**kw)
 
 
-def config4(mod, N=1024, nZ=2000):
+def config4_parts(N=1024, nZ=2000):
     """Differentiable Chi2PDC: N=1024, nZ=2000, tMax=20, Gaussian pump (w=1) with a
     seeded spectral phase phi ~ N(0, 0.5^2), per-step poling p ~ U(-1, 1).
 
-    Returns (medium, spectrum0, phi, poling): the pump envelope is
+    Returns (kwargs, spectrum0, phi, poling): the pump envelope is
     IFFT(spectrum0 * exp(i phi)) with spectrum0 = FFT(exp(-tau^2/2))."""
     r = np.random.default_rng(SEED0 + 4)
     b1s = float(r.uniform(-2, 2))
@@ -168,9 +168,18 @@ def config4(mod, N=1024, nZ=2000):
     tMax = 20.0
     tau = 2 * tMax / N * np.fft.fftshift(np.arange(-N // 2, N // 2))
     spec0 = np.fft.fft(np.exp(-tau ** 2 / 2).astype(np.complex128))
+    kw = dict(relativeLength=1.0, nlLength=1.0, beta2=b2, beta2s=b2s, beta1=0.0, beta1s=b1s, diffBeta0=0.0,
+              tMax=tMax, tPrecision=N, nZSteps=nZ)
+    return kw, spec0, phi, pol
+
+
+def config4(mod, N=1024, nZ=2000):
+    """The differentiable Chi2PDC medium of config4_parts with its pump and poling set.
+
+    Returns (medium, spectrum0, phi, poling)."""
+    kw, spec0, phi, pol = config4_parts(N, nZ)
     env = np.fft.ifft(spec0 * np.exp(1j * phi))
-    m = mod.Chi2PDC(relativeLength=1.0, nlLength=1.0, beta2=b2, beta2s=b2s, beta1=0.0, beta1s=b1s, diffBeta0=0.0,
-                    tMax=tMax, tPrecision=N, nZSteps=nZ, customPump=env, polingProfile=list(pol))
+    m = mod.Chi2PDC(customPump=env, polingProfile=list(pol), **kw)
     return m, spec0, phi, pol
 
 
@@ -207,3 +216,87 @@ def config3_instances(count=2000, seed=SEED0 + 3, sizes=(256, 512, 1024, 2048, 4
             x = np.exp(-tau ** 2 / 2).astype(np.complex128)
         out.append((name, kw, x))
     return out
+
+
+def config4_adjoint(m, spec0, phi, pol, r=1.5, ckpt_every=0):
+    """Loss ||S S^T - r I||_F^2 of a config4 medium (pump phase phi, poling pol
+    already set) and its gradient from the device adjoint: (L, dL/dphi, dL/dpol).
+    The envelope gradient is chained to the phase through env = IFFT(X),
+    X = spec0 exp(i phi)."""
+    N = spec0.size
+    C, S = m.greensForward(False, True, ckpt_every)
+    M = S @ S.T - r * np.eye(N)
+    gS = 4.0 * (M @ np.conj(S))
+    g_env, g_pol = m.greensAdjoint(np.zeros_like(C), gS)
+    X = spec0 * np.exp(1j * phi)
+    gX = np.fft.fft(g_env) / N
+    g_phi = np.real(np.conj(gX) * 1j * X)
+    return float(np.sum(np.abs(M) ** 2)), g_phi, g_pol
+
+
+def config4_directions(count, N=1024, nZ=2000, seed=SEED0 + 5):
+    """Seeded unit directions in (phase, poling) space: a list of (dphi, dpol)."""
+    r = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        d = r.standard_normal(N + nZ)
+        d /= np.linalg.norm(d)
+        out.append((d[:N], d[N:]))
+    return out
+
+
+def config4_loss_difference(m, spec0, plus, minus, r=1.5):
+    """L(plus) - L(minus) for (phi, pol) pairs on the device forward, evaluated as
+    Re sum conj(M+ + M-) (P+ - P-) with P = S S^T, M = P - r I, so the constant
+    r I cancels exactly instead of in floating point."""
+    def s_at(ph, pl):
+        m.setPump(np.fft.ifft(spec0 * np.exp(1j * ph)))
+        m.setPolingProfile(pl)
+        return m.computeGreensFunction()[1]
+
+    Sp, Sm = s_at(*plus), s_at(*minus)
+    Pp, Pm = Sp @ Sp.T, Sm @ Sm.T
+    I = np.eye(spec0.size)
+    return float(np.real(np.sum(np.conj(Pp - r * I + Pm - r * I) * (Pp - Pm))))
+
+
+def config4_fd(m, spec0, phi, pol, dphi, dpol, eps):
+    """Central difference of the config4 loss along (dphi, dpol) with step eps."""
+    return config4_loss_difference(m, spec0, (phi + eps * dphi, pol + eps * dpol),
+                                   (phi - eps * dphi, pol - eps * dpol)) / (2 * eps)
+
+
+def config3_check_sample(insts, random_count=16, seed=SEED0 + 30):
+    """Indices of the config-3 instances checked against the oracle: every Chi2PDC
+    instance of nFreqs 4096 with more than 3000 z steps (the longest in-kernel pump
+    generation and the batch's critical path), the longest instance of every
+    (model, nFreqs) class, and `random_count` others drawn uniformly."""
+    pick = set()
+    longest = {}
+    for i, (name, kw, _) in enumerate(insts):
+        key = (name, kw["tPrecision"])
+        if name == "Chi2PDC" and kw["tPrecision"] == 4096 and kw["nZSteps"] > 3000:
+            pick.add(i)
+        if key not in longest or kw["nZSteps"] > insts[longest[key]][1]["nZSteps"]:
+            longest[key] = i
+    pick.update(longest.values())
+    r = np.random.default_rng(seed)
+    rest = [i for i in range(len(insts)) if i not in pick]
+    pick.update(int(i) for i in r.choice(rest, size=min(random_count, len(rest)), replace=False))
+    return sorted(pick)
+
+
+def oracle_outputs(O, insts, indices, workers=None):
+    """The oracle's batchSignalSimulation of the given config-3 instances (time-domain
+    input), computed on a pool of host threads (the oracle releases the GIL):
+    {index: output row}."""
+    import concurrent.futures as cf
+    import os
+
+    def one(i):
+        name, kw, x = insts[i]
+        return i, getattr(O, name)(**kw).batchSignalSimulation(x[None, :], inTimeDomain=True)[0]
+
+    workers = workers or min(16, os.cpu_count() or 1)
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        return dict(ex.map(one, indices))

@@ -138,7 +138,8 @@ def test_adjoint_errors(nlm):
         sfg.greensForward(False, True, 0)
 
 
-@pytest.mark.parametrize("N,nZ,ckpt", [(1024, 33, 0), (2048, 17, 4), (256, 50, 7), (4096, 9, 3), (8192, 5, 2)])
+@pytest.mark.parametrize("N,nZ,ckpt", [(1024, 33, 0), (2048, 17, 4), (2048, 50, 0), (256, 50, 7), (4096, 9, 3),
+                                       (8192, 5, 2)])
 def test_gradients_match_fd_of_device_forward(nlm, N, nZ, ckpt):
     """Larger sizes (one and two fields per workgroup in the reverse sweep, and
     512 / 1024 lanes per field from N = 4096): the finite differences come from
@@ -164,3 +165,38 @@ def test_gradients_match_fd_of_device_forward(nlm, N, nZ, ckpt):
         fd = (L(env + eps * de, pol + eps * dp) - L(env - eps * de, pol - eps * dp)) / (2 * eps)
         ad = float(np.real(np.vdot(g_env, de)) + np.dot(g_pol, dp))
         assert abs(ad - fd) <= 1e-6 * max(abs(fd), 1e-3), (ad, fd)
+
+
+def test_adjoint_refuses_a_changed_medium(nlm):
+    """The adjoint reads the checkpoints and rows of the latest greensForward and the
+    medium's current tables: a Green's run or a coefficient change in between must
+    raise instead of mixing two forwards."""
+    N, nZ = 64, 20
+    kw, env, pol = _kwargs(N, nZ, SEED0 + 6)
+    m = nlm.Chi2PDC(customPump=env, **kw)
+    m.setPolingProfile(pol)
+    g = (np.ones((N, N), complex), np.ones((N, N), complex))
+    m.greensForward(False, True, 0)
+    ref = m.greensAdjoint(*g)
+    m.computeGreensFunction()
+    with pytest.raises(RuntimeError):
+        m.greensAdjoint(*g)
+    m.greensForward(False, True, 0)
+    m.setPolingProfile(pol[::-1].copy())
+    with pytest.raises(RuntimeError):
+        m.greensAdjoint(*g)
+    m.setPolingProfile(pol)
+    m.greensForward(False, True, 0)
+    again = m.greensAdjoint(*g)
+    assert np.array_equal(again[0], ref[0]) and np.array_equal(again[1], ref[1])
+    # two autograd forwards on one medium: the first graph's backward must refuse
+    import torch
+    from multimodenonlinearoptics_amd.autograd import GreensFunction
+    e = torch.tensor(env, requires_grad=True)
+    p = torch.tensor(pol, requires_grad=True)
+    _, S1 = GreensFunction.apply(m, e, p)
+    _, S2 = GreensFunction.apply(m, e, p)
+    with pytest.raises(RuntimeError):
+        S1.abs().sum().backward()
+    S2.abs().sum().backward()
+    assert e.grad is not None and torch.isfinite(e.grad).all()

@@ -89,3 +89,26 @@ def test_mixed_batch_errors(nlm):
         nlm.MultiBatch([pdc], [np.zeros(60, complex)])
     with pytest.raises(ValueError):
         nlm.MultiBatch([pdc, pdc], [np.zeros(64, complex)])
+
+
+def test_config3_workload_matches_oracle(nlm, oracle):
+    """The heterogeneous-batch configuration at its workload: 2000 Chi2SHG /
+    Chi2PDC media (nFreqs 256..4096, nZSteps 200..4000) in one graph launch, as the
+    benchmark runs it.  Checked against the oracle on every Chi2PDC medium of
+    nFreqs 4096 with more than 3000 steps (in-kernel pump rows over up to 8000 rows),
+    the longest medium of every class and 16 others."""
+    from cases import config3_check_sample, oracle_outputs
+    insts = config3_instances(2000)
+    idx = config3_check_sample(insts)
+    print("config 3: %d media, %d checked (longest nZ %d)" % (len(insts), len(idx),
+                                                              max(insts[i][1]["nZSteps"] for i in idx)), flush=True)
+    media = _build(nlm, insts)
+    b = nlm.MultiBatch(media, [x for _, _, x in insts], inTimeDomain=True, autotune=True)
+    b.run()
+    exp = oracle_outputs(oracle, insts, idx)
+    worst = 0.0
+    for i in idx:
+        err = rel_l2(b.output(i), exp[i])
+        worst = max(worst, err)
+        assert err < TOL, (i, insts[i][0], insts[i][1]["tPrecision"], insts[i][1]["nZSteps"], err)
+    print("config 3: worst rel. L2 %.2e" % worst, flush=True)
