@@ -130,3 +130,32 @@ def reverse_gradient(T, phi, pol, r=1.5):
     L = squeezing_loss(T.greens(ph, pl)[1], r)
     L.backward()
     return float(L.detach()), ph.grad.cpu().numpy(), pl.grad.cpu().numpy()
+
+
+def run_directions(kw, spec0, phi, pol, dirs, device):
+    """[(L, dL)] for each (dphi, dpol) in dirs."""
+    T = PdcGreens(kw, spec0, device=device)
+    out = []
+    for k, (dphi, dpol) in enumerate(dirs):
+        out.append(loss_and_directional_derivative(T, phi, pol, dphi, dpol))
+        print("forward-mode direction %d: L=%.15e dL=%.15e" % (k, out[-1][0], out[-1][1]), flush=True)
+    return out
+
+
+def main(argv):
+    """python pdc_torch.py IN.npz OUT.npz: forward-mode directional derivatives on
+    the GPU in a process of its own.  (The engine links the system HIP runtime and
+    torch its bundled copy; the two do not share a device within one process.)
+    IN holds kw (JSON text), spec0, phi, pol, dphi [K, N], dpol [K, nZ]."""
+    import json
+    d = np.load(argv[1])
+    kw = json.loads(str(d["kw"]))
+    dirs = list(zip(d["dphi"], d["dpol"]))
+    res = run_directions(kw, d["spec0"], d["phi"], d["pol"], dirs, "cuda")
+    np.savez(argv[2], L=np.array([r[0] for r in res]), dL=np.array([r[1] for r in res]))
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main(sys.argv))

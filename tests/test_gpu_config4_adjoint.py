@@ -26,21 +26,29 @@ def adjoint(nlm):
     return m, spec0, phi, pol, L, g_phi, g_pol
 
 
-def test_adjoint_matches_forward_mode_derivative(adjoint):
-    import torch
-    import pdc_torch as PT
+def test_adjoint_matches_forward_mode_derivative(adjoint, tmp_path):
+    import json
+    import os
+    import subprocess
+    import sys
     m, spec0, phi, pol, L, g_phi, g_pol = adjoint
     kw = config4_parts()[0]
-    T = PT.PdcGreens(kw, spec0, device="cuda")
-    for k, (dphi, dpol) in enumerate(config4_directions(3)):
-        Lt, dL = PT.loss_and_directional_derivative(T, phi, pol, dphi, dpol)
+    dirs = config4_directions(3)
+    src, dst = tmp_path / "in.npz", tmp_path / "out.npz"
+    np.savez(src, kw=json.dumps(kw), spec0=spec0, phi=phi, pol=pol, dphi=np.array([d[0] for d in dirs]),
+             dpol=np.array([d[1] for d in dirs]))
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run([sys.executable, os.path.join(here, "pdc_torch.py"), str(src), str(dst)], timeout=240,
+                       capture_output=True, text=True)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    res = np.load(dst)
+    for k, ((dphi, dpol), Lt, dL) in enumerate(zip(dirs, res["L"], res["dL"])):
         ad = float(np.dot(g_phi, dphi) + np.dot(g_pol, dpol))
         print("direction %d: adjoint %.12e forward-mode %.12e rel %.2e" % (k, ad, dL, abs(ad - dL) / abs(dL)),
               flush=True)
         assert abs(Lt - L) <= 1e-10 * abs(L)
         assert abs(ad - dL) <= 1e-6 * abs(dL), (k, ad, dL)
-    del T
-    torch.cuda.empty_cache()
 
 
 def test_adjoint_matches_extrapolated_differences(adjoint):
