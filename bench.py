@@ -5,20 +5,32 @@ tMax=20, nZSteps=1000, seeded Gaussian pump (width ~ U(0.5, 2)) and dispersion
 (beta1, beta1s ~ U(-2, 2); beta2, beta2s ~ U(-1, 1); diffBeta0 ~ U(-5, 5)),
 uniform poling.  One step = one computeGreensFunction(): the pump grid
 (2nZ-1 rows), the 2N impulse propagations of nZ-1 split steps each in one
-launch, and the C/S assembly, with C and S left in HBM (the host copy is not
-timed).  Metric: field-steps/s, a field-step being one propagation advanced by
-one RK4 step plus one dispersion step (2*512*999 per solve).
+launch, and the C/S assembly, with C and S left in HBM (the host copy is timed
+separately as end_to_end).  Metric: field-steps/s, a field-step being one
+propagation advanced by one RK4 step plus one dispersion step (2*512*999 per
+solve).
 
 Reported beside it:
-  plain      the same kernel run as one propagation per launch, serially
-             (sampled on --plain-sample impulses, scaled to the full solve);
-  roofline   the propagation kernel's achieved FP64 rate (algorithmic flops
-             10 N log2 N + 80 N per field-step) against the 78.6 TFLOP/s peak,
-             from HIP events on the kernel's own stream;
+  plain         the same kernel run as one propagation per launch, serially
+                (sampled on --plain-sample impulses, scaled to the full solve);
+  end_to_end    computeGreensFunction() including the copy of C and S to the host;
+  roofline      the propagation kernel's achieved FP64 rate (algorithmic flops
+                10 N log2 N + 80 N per field-step) against the 78.6 TFLOP/s peak,
+                from HIP events on the kernel's own stream;
   cpu_baseline  the C oracle (a restatement of the reference C++ path with a
-             kissfft-class FFT and the reference's std::thread fan-out) timed on
-             this host on the same Green's solve;
-  rel_err    relative L2 difference of (C, S) against that CPU result.
+                kissfft-class FFT and the reference's std::thread fan-out), compiled
+                on this host with the reference's release flags, timed on every core
+                this process may use (at most 16) and on one thread, median of 5 after
+                a warm-up, with the host's CPU model and core counts;
+  rel_err       relative L2 difference of (C, S) against that CPU result;
+  config1       the plain Chi3 solve (pump and signal z-grids, N=512, nZ=1000);
+  hetero        2000 heterogeneous Chi2SHG/Chi2PDC media in one graph launch;
+  gnlse         the Chi3GNLSE batch at nFreqs=16384 through HBM-staged transforms;
+  gnlse_grad    its input gradient by the adjoint;
+  compose       the Cascade Green's composition on the FP64 matrix cores;
+  grad          the differentiable Chi2PDC (N=1024, nZ=2000): adjoint cost and its
+                directional derivatives against finite differences and torch
+                forward-mode AD.
 """
 import argparse
 import json
@@ -46,41 +58,27 @@ def flops_per_field_step(n):
     return 10 * n * math.log2(n) + 80 * n
 
 
-def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6):
+def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4):
     """Differentiable Chi2PDC (N=1024, nZ=2000): loss ||S S^T - 1.5 I||_F^2 of the
     Green's function S, gradient w.r.t. the pump spectral phase (1024 reals) and the
     per-step poling (2000 reals) by the adjoint kernel with ~sqrt(nZ) checkpoints.
 
     Reports the backward time over one forward Green's solve (no checkpoints), both
-    as wall time of the calls and as kernel time, and the worst relative error of
-    the directional derivative against central finite differences of the device
-    forward along n_dirs seeded unit directions."""
+    as wall time of the calls and as kernel time, and the directional derivative
+    along n_dirs seeded unit directions against three references:
+      fd_eps       central differences of the device forward with step 1e-6 (their
+                   rounding floor here is ~1e-4 of the derivative: the loss is 2304
+                   and a step of 1e-6 moves it by ~1e-12 of itself);
+      fd_rich      Richardson extrapolation of central differences at steps 4e-3
+                   and 2e-3 (truncation O(eps^4), rounding ~1e-13 / eps);
+      forward mode torch forward-mode AD of the torch restatement (tests/pdc_torch.py,
+                   pinned to the oracle) on fwd_mode_dirs directions, run in a
+                   process of its own."""
     import numpy as np
-    from cases import SEED0, config4
+    from cases import config4, config4_adjoint, config4_directions, config4_fd, config4_parts
     m, spec0, phi, pol = config4(nlm)
     N, nZ = spec0.size, pol.size
     m.enableTiming(True)
-
-    def env_of(ph):
-        return np.fft.ifft(spec0 * np.exp(1j * ph))
-
-    def loss_and_gS(S):
-        M = S @ S.T - 1.5 * np.eye(N)
-        return float(np.sum(np.abs(M) ** 2)), 4.0 * (M @ np.conj(S))
-
-    def s_at(ph, pl):
-        m.setPump(env_of(ph))
-        m.setPolingProfile(pl)
-        return m.computeGreensFunction()[1]
-
-    def loss_diff(ph_p, pl_p, ph_m, pl_m):
-        # L+ - L- = Re sum conj(M+ + M-) (M+ - M-), with M+ - M- = S+ S+^T - S- S-^T:
-        # the constant r I cancels exactly instead of in floating point
-        Sp, Sm = s_at(ph_p, pl_p), s_at(ph_m, pl_m)
-        Pp, Pm = Sp @ Sp.T, Sm @ Sm.T
-        Mp, Mm = Pp - 1.5 * np.eye(N), Pm - 1.5 * np.eye(N)
-        return float(np.real(np.sum(np.conj(Mp + Mm) * (Pp - Pm))))
-
     m.greensOnDevice()
     m.sync()
     t_fwd, k_fwd = [], []
@@ -95,30 +93,30 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6):
         t0 = time.perf_counter()
         C, S = m.greensForward(False, True, 0)
         t_ck.append(time.perf_counter() - t0)
-        L0, gS = loss_and_gS(S)
+        M = S @ S.T - 1.5 * np.eye(N)
+        gS = 4.0 * (M @ np.conj(S))
         t0 = time.perf_counter()
-        g_env, g_pol = m.greensAdjoint(np.zeros_like(C), gS)
+        m.greensAdjoint(np.zeros_like(C), gS)
         t_bwd.append(time.perf_counter() - t0)
         tm = m.lastTiming()
         k_bwd.append(tm[1] + tm[2])
-    # chain rule envelope -> spectral phase: env = IFFT(X), X = spec0 exp(i phi)
-    X = spec0 * np.exp(1j * phi)
-    gX = np.fft.fft(g_env) / N
-    g_phi = np.real(np.conj(gX) * 1j * X)
-    r = np.random.default_rng(SEED0 + 5)
-    worst = 0.0
-    pairs = []
-    for _ in range(n_dirs):
-        d = r.standard_normal(N + nZ)
-        d /= np.linalg.norm(d)
-        dphi, dpol = d[:N], d[N:]
-        fd = loss_diff(phi + fd_eps * dphi, pol + fd_eps * dpol, phi - fd_eps * dphi, pol - fd_eps * dpol) / (2 * fd_eps)
-        ad = float(np.dot(g_phi, dphi) + np.dot(g_pol, dpol))
-        worst = max(worst, abs(ad - fd) / max(abs(fd), 1e-30))
-        pairs.append((ad, fd))
-    m.setPump(env_of(phi))
+    L0, g_phi, g_pol = config4_adjoint(m, spec0, phi, pol)
+    dirs = config4_directions(n_dirs, N, nZ)
+    ads = [float(np.dot(g_phi, dp) + np.dot(g_pol, dq)) for dp, dq in dirs]
+    raw, rich = [], []
+    for dp, dq in dirs:
+        raw.append(config4_fd(m, spec0, phi, pol, dp, dq, fd_eps))
+        f2 = config4_fd(m, spec0, phi, pol, dp, dq, 2e-3)
+        f4 = config4_fd(m, spec0, phi, pol, dp, dq, 4e-3)
+        rich.append((4 * f2 - f4) / 3)
+    m.setPump(np.fft.ifft(spec0 * np.exp(1j * phi)))
     m.setPolingProfile(pol)
-    return {
+
+    def rel(a, b):
+        return abs(a - b) / max(abs(b), 1e-300)
+
+    fwd = forward_mode_reference(config4_parts(N, nZ)[0], spec0, phi, pol, dirs[:fwd_mode_dirs])
+    out = {
         "workload": "Chi2PDC N=1024 nZ=2000, loss ||S S^T - 1.5 I||_F^2, grad w.r.t. pump spectral phase "
                     "(1024) and per-step poling (2000)",
         "loss": L0,
@@ -126,9 +124,40 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6):
         "forward_kernel_ms": min(k_fwd), "backward_kernel_ms": min(k_bwd),
         "grad_vs_forward": min(t_bwd) / min(t_fwd),
         "grad_vs_forward_kernel": min(k_bwd) / min(k_fwd),
-        "fd_directions": n_dirs, "fd_eps": fd_eps, "fd_max_rel_err": worst,
-        "fd_pairs": pairs[:6], "grad_norm": float(np.sqrt(np.sum(g_phi ** 2) + np.sum(g_pol ** 2))),
+        "directions": n_dirs,
+        "fd_rich_max_rel_err": max(rel(a, r) for a, r in zip(ads, rich)),
+        "fd_rich_steps": [4e-3, 2e-3],
+        "fd_eps": fd_eps,
+        "fd_eps_max_rel_err": max(rel(a, f) for a, f in zip(ads, raw)),
+        "fd_eps_noise_floor": max(rel(f, r) for f, r in zip(raw, rich)),
+        "pairs": [[a, r, f] for a, r, f in zip(ads[:6], rich[:6], raw[:6])],
+        "grad_norm": float(np.sqrt(np.sum(g_phi ** 2) + np.sum(g_pol ** 2))),
     }
+    if fwd is not None:
+        out["forward_mode_directions"] = len(fwd)
+        out["forward_mode_max_rel_err"] = max(rel(a, d) for a, (_, d) in zip(ads, fwd))
+        out["forward_mode_loss_rel_err"] = max(rel(Lt, L0) for Lt, _ in fwd)
+    return out
+
+
+def forward_mode_reference(kw, spec0, phi, pol, dirs, timeout=300):
+    """[(L, dL)] along dirs from torch forward-mode AD of tests/pdc_torch.py, on the
+    GPU in a child process (torch's bundled HIP runtime and the engine's do not
+    share a device within one process); None if that fails."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    with tempfile.TemporaryDirectory() as d:
+        src, dst = os.path.join(d, "in.npz"), os.path.join(d, "out.npz")
+        np.savez(src, kw=json.dumps(kw), spec0=spec0, phi=phi, pol=pol, dphi=np.array([x[0] for x in dirs]),
+                 dpol=np.array([x[1] for x in dirs]))
+        r = subprocess.run([sys.executable, os.path.join(HERE, "tests", "pdc_torch.py"), src, dst], timeout=timeout,
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            print("forward-mode reference failed:\n" + r.stdout[-2000:], file=sys.stderr)
+            return None
+        res = np.load(dst)
+        return list(zip(res["L"].tolist(), res["dL"].tolist()))
 
 
 def flops_shg(n):
@@ -136,14 +165,16 @@ def flops_shg(n):
     return 2 * (10 * n * math.log2(n) + 8 * n) + 154 * n
 
 
-def hetero_benchmark(nlm, count=2000, reps=3, plain_sample=40, cpu_sample=3):
+def hetero_benchmark(nlm, count=2000, reps=3, plain_sample=40, cpu_sample=8, cpu_cores=1):
     """Heterogeneous batch: `count` Chi2SHG / Chi2PDC media of mixed nFreqs (256..4096)
     and nZSteps (200..4000), one input row each, run as one MultiBatch graph launch.
 
     Reports field-steps/s of the whole batch (inputs resident in HBM, device time of
     the graph launch), the same instances run one by one through
-    batchSignalSimulation (sampled, scaled by algorithmic cost), the C oracle on a
-    few instances, and the worst rel. L2 error of sampled outputs against it."""
+    batchSignalSimulation (sampled, scaled by algorithmic cost), and the C oracle on
+    the cpu_sample costliest instances (N = 4096, nZ near 4000) plus cpu_sample random
+    ones, on cpu_cores host threads: its throughput and the worst rel. L2 error of
+    those outputs against it."""
     import numpy as np
     from cases import config3_instances
     from oracle import oracle as O
@@ -172,18 +203,15 @@ def hetero_benchmark(nlm, count=2000, reps=3, plain_sample=40, cpu_sample=3):
     for i in idx:
         media[i].batchSignalSimulation(xs[i][None, :], inTimeDomain=True)
     plain_s = (time.perf_counter() - p0) * total_flops / sum(cost[i] for i in idx)
-    # oracle on the cheapest instances: parity and CPU throughput
-    cheap = sorted(range(count), key=lambda i: cost[i])[:cpu_sample]
-    worst, cpu_t, cpu_steps = 0.0, 0.0, 0
-    for i in cheap:
-        name, kw, x = insts[i]
-        ref = getattr(O, name)(**kw)
-        c0 = time.perf_counter()
-        exp = ref.batchSignalSimulation(x[None, :], inTimeDomain=True)[0]
-        cpu_t += time.perf_counter() - c0
-        cpu_steps += steps[i]
-        got = b.output(i)
-        worst = max(worst, float(np.linalg.norm(got - exp) / np.linalg.norm(exp)))
+    # oracle on the costliest instances and a random sample: parity and CPU throughput
+    from cases import oracle_outputs
+    order = sorted(range(count), key=lambda i: -cost[i])
+    pick = sorted(set(order[:cpu_sample]) | set(r.choice(count, size=cpu_sample, replace=False).tolist()))
+    c0 = time.perf_counter()
+    exp = oracle_outputs(O, insts, pick, workers=cpu_cores)
+    cpu_t = time.perf_counter() - c0
+    cpu_steps = sum(steps[i] for i in pick)
+    worst = max(float(np.linalg.norm(b.output(i) - exp[i]) / np.linalg.norm(exp[i])) for i in pick)
     classes = [list(c) for c in b.classes()]
     return {
         "workload": "%d Chi2SHG/Chi2PDC instances, nFreqs in {256..4096}, nZSteps U[200,4000], one graph launch"
@@ -193,8 +221,10 @@ def hetero_benchmark(nlm, count=2000, reps=3, plain_sample=40, cpu_sample=3):
         "value": total_steps / (ms * 1e-3), "unit": "field-steps/s",
         "tflops": total_flops / (ms * 1e-3) / 1e12,
         "plain_s": plain_s, "plain_sample": len(idx), "batched_vs_plain": plain_s / (ms * 1e-3),
-        "cpu": {"field_steps_per_s": cpu_steps / cpu_t, "cores": 1, "kind": "port",
-                "sample": "%d cheapest instances on one thread" % len(cheap)},
+        "cpu": {"field_steps_per_s": cpu_steps / cpu_t, "cores": cpu_cores, "kind": "port",
+                "sample": "%d instances (the %d costliest and %d random) on %d threads, %.1f s"
+                          % (len(pick), cpu_sample, cpu_sample, cpu_cores, cpu_t)},
+        "rel_err_sample": len(pick),
         "rel_err_max": worst, "classes": classes,
     }
 
@@ -359,6 +389,84 @@ def compose_benchmark(n=512, reps=20):
             "frac_fp64_peak": flops / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, "rel_err": err}
 
 
+def host_cpu():
+    """The host's CPU model and count, and the cores this process may use: its
+    affinity mask, capped by a cgroup CPU quota and by the 16-core share a GPU box
+    gives each GPU."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()
+        if q != "max":
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    cores = min(aff, max(1, int(quota)) if quota else aff, 16)
+    return {"model": model, "nproc": os.cpu_count(), "affinity": aff, "cgroup_quota": quota, "cores": cores}
+
+
+def median_time(fn, reps=5):
+    """Median wall time of fn() over reps runs after one warm-up run."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def config1_benchmark(nlm, O, reps=5):
+    """Plain Chi3 (NLSE) solve: N=1 soliton pump, nFreqs=512, tMax=10, zPrecision 637
+    (nZSteps 1000), runPumpSimulation (1998 serial nonlinear pump steps) then
+    runSignalSimulation of a seeded Gaussian signal (999 steps), the full pump
+    (1999 x 512) and signal (1000 x 512) grids compared with the oracle.  Times are
+    wall clock of the two calls (grids left on the device) and with the four grids
+    read back; the oracle runs the same calls on one thread (the path is serial)."""
+    import numpy as np
+    from cases import config1
+    fiber, sig = config1(nlm)
+    steps = (2 * fiber.nZSteps - 2) + (fiber.nZSteps - 1)
+
+    def run():
+        fiber.runPumpSimulation()
+        fiber.runSignalSimulation(sig)
+        fiber.sync()
+
+    def run_read():
+        run()
+        return [np.asarray(g) for g in (fiber.pumpTime, fiber.pumpFreq, fiber.signalTime, fiber.signalFreq)]
+
+    t_dev = median_time(run, reps)
+    t_e2e = median_time(run_read, reps)
+    got = run_read()
+    ref, rsig = config1(O)
+
+    def run_ref():
+        ref.runPumpSimulation()
+        ref.runSignalSimulation(rsig)
+
+    t_cpu = median_time(run_ref, reps)
+    exp = [np.asarray(g) for g in (ref.pumpTime, ref.pumpFreq, ref.signalTime, ref.signalFreq)]
+    errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(got, exp)]
+    return {"workload": "Chi3 soliton pump + Gaussian signal, nFreqs=512, nZSteps=1000, full z-resolved grids",
+            "field_steps": steps, "s": t_dev, "value": steps / t_dev, "unit": "field-steps/s",
+            "s_with_grid_download": t_e2e, "cpu": {"s": t_cpu, "field_steps_per_s": steps / t_cpu, "cores": 1,
+                                                   "kind": "port", "sample": "the whole run, median of 5"},
+            "vs_cpu": t_cpu / t_dev,
+            "rel_err": {"pumpTime": errs[0], "pumpFreq": errs[1], "signalTime": errs[2], "signalFreq": errs[3]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -372,11 +480,13 @@ def main():
     ap.add_argument("--no-gnlse", action="store_true", help="skip the nFreqs=16384 Chi3GNLSE measurement")
     ap.add_argument("--no-gnlse-grad", action="store_true", help="skip the Chi3GNLSE input-gradient measurement")
     ap.add_argument("--no-compose", action="store_true", help="skip the Green's composition measurement")
-    ap.add_argument("--only", choices=["hetero", "gnlse", "grad", "gnlse_grad", "compose"], default=None,
+    ap.add_argument("--no-config1", action="store_true", help="skip the plain Chi3 solve measurement")
+    ap.add_argument("--only", choices=["hetero", "gnlse", "grad", "gnlse_grad", "compose", "config1"], default=None,
                     help="run only this section and print its record (for profiling)")
     ap.add_argument("--gnlse-nz", type=int, default=20000,
                     help="nZSteps of the Chi3GNLSE section (the medium is shortened in proportion)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (default: min(16, cores))")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads for the CPU baseline (default: every core this process may use, at most 16)")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="only run a device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -398,11 +508,27 @@ def main():
     import numpy as np
     import multimodenonlinearoptics_amd as pkg
     from cases import config2
+    from oracle import oracle as O
+
+    host = host_cpu()
+    cores = args.cpu_threads or host["cores"]
+    # the CPU leg runs the oracle compiled here with the reference's release flags
+    oracle_build = {"flags": "in-tree (-O3 -march=x86-64-v3, no contraction)"}
+    if not args.no_cpu:
+        import tempfile
+        path = os.path.join(tempfile.mkdtemp(prefix="nlm_oracle_"), "libnlm_oracle_native.so")
+        try:
+            oracle_build = {"flags": " ".join(O.build_native(path)[1:7])}
+            O.use_library(path)
+        except Exception as e:  # keep the in-tree build
+            oracle_build["native_build_error"] = str(e)[:200]
 
     nlm = pkg.nonlinearmedium
     if args.only:
         if args.only == "hetero":
-            out = hetero_benchmark(nlm, count=args.hetero_count)
+            out = hetero_benchmark(nlm, count=args.hetero_count, cpu_cores=cores)
+        elif args.only == "config1":
+            out = config1_benchmark(nlm, O)
         elif args.only == "gnlse":
             out = gnlse_benchmark(nlm, nz=args.gnlse_nz)
         elif args.only == "gnlse_grad":
@@ -435,6 +561,8 @@ def main():
     value = field_steps / (ms_per_step * 1e-3)
 
     prop_avg = sum(prop_ms) / len(prop_ms)
+    # end to end: computeGreensFunction with C and S copied to host numpy arrays
+    e2e_s = median_time(lambda: medium.computeGreensFunction(), 5)
     fl = flops_per_field_step(N_FREQS)
     achieved_tflops = fl * field_steps / (prop_avg * 1e-3) / 1e12
 
@@ -475,6 +603,8 @@ def main():
         "plain": {"value": plain_value, "unit": "field-steps/s", "sample_propagations": sample,
                   "s_per_solve": plain_solve_s},
         "batched_vs_plain": value / plain_value if plain_value else None,
+        "end_to_end": {"s_per_solve": e2e_s, "value": field_steps / e2e_s, "unit": "field-steps/s",
+                       "note": "computeGreensFunction() with C and S (2 x 4 MB) copied to host, median of 5"},
         "grad_vs_forward": None,
         "grad": None,
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
@@ -482,8 +612,11 @@ def main():
                      "kernel": "k_prop_tab<9> (Chi2PDC Green's launch)", "flops_per_field_step": fl},
     }
 
+    if not args.no_config1:
+        rec["config1"] = config1_benchmark(nlm, O)
+
     if not args.no_hetero:
-        rec["hetero"] = hetero_benchmark(nlm, count=args.hetero_count)
+        rec["hetero"] = hetero_benchmark(nlm, count=args.hetero_count, cpu_cores=cores)
 
     if not args.no_gnlse:
         rec["gnlse"] = gnlse_benchmark(nlm, nz=args.gnlse_nz)
@@ -501,20 +634,28 @@ def main():
         rec["grad_vs_forward"] = g["grad_vs_forward"]
 
     if not args.no_cpu:
-        from oracle import oracle as O
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         ref = config2(O, N_FREQS, NZ)
-        tc0 = time.perf_counter()
-        Co, So = ref.computeGreensFunction(nThreads=threads)
-        tc1 = time.perf_counter()
-        cpu_value = field_steps / (tc1 - tc0)
+        t_all = median_time(lambda: ref.computeGreensFunction(nThreads=cores), 5)
+        Co, So = ref.computeGreensFunction(nThreads=cores)
+        # one thread: 32 of the 1024 impulse propagations of the same solve
+        ref.runPumpSimulation()
+        X = np.zeros((32, N_FREQS), dtype=np.complex128)
+        for j in range(32):
+            X[j, (j * 37) % N_FREQS] = 1.0 if j % 2 == 0 else 1j
+        t_one = median_time(lambda: ref.batchSignalSimulation(X, inTimeDomain=False, runPump=False, nThreads=1), 5)
+        one_value = 32 * (NZ - 1) / t_one
         C, S = medium.computeGreensFunction()
         num = np.linalg.norm(np.hstack([C - Co, S - So]))
         den = np.linalg.norm(np.hstack([Co, So]))
-        rec["cpu_baseline"] = {"value": cpu_value, "unit": "field-steps/s", "cores": threads, "kind": "port",
-                               "sample": "one full Green's solve (2x512 propagations x 999 steps) on %d threads, "
-                                         "%.1f s" % (threads, tc1 - tc0)}
+        cpu_value = field_steps / t_all
+        rec["cpu_baseline"] = {"value": cpu_value, "unit": "field-steps/s", "cores": cores, "kind": "port",
+                               "sample": "the full Green's solve (2x512 propagations x 999 steps) on %d threads, "
+                                         "median of 5 after a warm-up: %.3f s" % (cores, t_all),
+                               "value_1thread": one_value,
+                               "sample_1thread": "32 of its propagations on one thread, median of 5: %.3f s" % t_one,
+                               "host": host, "build": oracle_build}
         rec["vs_cpu"] = value / cpu_value
+        rec["vs_cpu_1thread"] = value / one_value
         rec["rel_err"] = float(num / den)
     print(json.dumps(rec))
 

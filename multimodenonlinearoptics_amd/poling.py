@@ -1,8 +1,9 @@
 """Poling design helpers that feed the per-step poling of the engine
 (``polingProfile=`` / ``setPolingProfile``) and take an optimised profile back to
-domains, after the reference's poling.py:8-15 and :439-470.
+domains, after the reference's poling.py:8-29 and :439-470.
 
   periodicPoling(deltaBeta0, L)      domain lengths of first-order quasi-phase matching
+  linearPoling(k0, kf, L, dL)        domains with a linearly swept phase-matching wavenumber
   domainsToSpace(poling, nZSteps)    domain lengths -> per-step orientation, the
                                      discretisation of _NonlinearMedium::setPoling
                                      (src/_NonlinearMedium.cpp:470-503)
@@ -29,6 +30,28 @@ def periodicPoling(deltaBeta0, L):
     if frac > 0:
         lengths[-1] = frac
     return lengths * (L / lengths.sum())
+
+
+def linearPoling(k0, kf, L, dL):
+    """Domains whose quasi-phase-matching wavenumber sweeps linearly from k0 at z = 0
+    to kf at z = L (poling.py:17-29 of the reference): the orientation is the sign of
+    sin(phi(z)), phi(z) = (kf - k0) z^2 / (2 L) + k0 z (whose rate k0 + (kf - k0) z / L
+    is the local wavenumber), sampled at round(L / dL) points from 0 to L inclusive
+    (a zero counts as +1), and the domains are the runs of equal sign, each sample
+    counting dL."""
+    n = int(round(L / dL))
+    z = np.linspace(0.0, L, n)
+    s = np.sign(np.sin(0.5 * (kf - k0) * z * z / L + k0 * z))
+    s[s == 0] = 1.0
+    return _run_lengths(s) * dL
+
+
+def _run_lengths(s):
+    """Lengths (in samples) of the runs of equal values of s."""
+    if s.size == 0:
+        return np.zeros(0)
+    edges = np.flatnonzero(s[1:] != s[:-1]) + 1
+    return np.diff(np.concatenate([[0], edges, [s.size]])).astype(np.float64)
 
 
 def domainsToSpace(poling, nZSteps):
@@ -71,6 +94,4 @@ def spaceToDomains(profile, length):
     for i in range(s.size):
         if s[i] == 0:
             s[i] = s[i - 1] if i > 0 else 1.0
-    edges = np.flatnonzero(s[1:] != s[:-1]) + 1
-    bounds = np.concatenate([[0], edges, [s.size]])
-    return np.diff(bounds) * (length / s.size), float(s[0])
+    return _run_lengths(s) * (length / s.size), float(s[0])

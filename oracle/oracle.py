@@ -53,6 +53,30 @@ def build(force=False):
     return _LIB_PATH
 
 
+# the reference's release flags (CMakeLists.txt:30; -fcx-limited-range stands for
+# Eigen's packet complex products, which skip the C99 NaN recovery)
+NATIVE_FLAGS = ["-O3", "-DNDEBUG", "-march=native", "-flto", "-fno-math-errno", "-fcx-limited-range"]
+
+
+def build_native(path):
+    """Compile nlm_oracle.c with the reference's release flags for the host it runs
+    on (used by the CPU-baseline timing; the in-tree build above, portable and
+    without contraction, is the parity reference).  Returns the gcc command."""
+    import subprocess
+    src = os.path.join(_HERE, "nlm_oracle.c")
+    cmd = ["gcc"] + NATIVE_FLAGS + ["-fPIC", "-shared", "-pthread", "-o", path, src, "-lm"]
+    subprocess.check_call(cmd)
+    return cmd
+
+
+def use_library(path):
+    """Load the oracle from `path` instead of the in-tree build (before first use)."""
+    global _LIB_PATH, _lib
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    _LIB_PATH = path
+
+
 class _Params(C.Structure):
     _fields_ = [
         ("model", C.c_int), ("n_signal_modes", C.c_int), ("n_pump_modes", C.c_int),
@@ -80,7 +104,8 @@ class _Params(C.Structure):
 def lib():
     global _lib
     if _lib is None:
-        build()
+        if _LIB_PATH == os.path.join(_HERE, "libnlm_oracle.so"):
+            build()
         L = C.CDLL(_LIB_PATH)
         dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_void_p
         u8 = C.POINTER(C.c_uint8)
@@ -520,6 +545,11 @@ class Cascade:
         self.connections = [dict(c) for c in modeConnections]
         self.sharedPump = bool(sharePump)
         self._N = N
+
+    def setPump(self, *args, **kw):
+        """src/Cascade.cpp:57-72: the first medium's pump when it is shared, else every medium's."""
+        for m in (self.media[:1] if self.sharedPump else self.media):
+            m.setPump(*args, **kw)
 
     def runPumpSimulation(self):
         if not self.sharedPump:

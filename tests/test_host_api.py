@@ -199,3 +199,20 @@ def test_grid_views_are_read_only_and_shaped(nlm):
     assert not m.signalFreq.flags.writeable
     with pytest.raises(IndexError):
         m.signalTimes(2)
+
+
+def test_reference_import_name_in_a_fresh_interpreter():
+    """`from nonlinearmedium import ...` works with only the repository on sys.path
+    (src/nlmModulePy.cpp:12), and construction (host-side set-up) needs no GPU."""
+    import subprocess
+    import sys
+    from conftest import REPO
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from nonlinearmedium import Chi3, Chi2PDC, Cascade, _NonlinearMedium, _FullyNonlinearMedium\n"
+            "import nonlinearmedium\n"
+            "m = Chi3(relativeLength=1.0, nlLength=1, beta2=-1, pulseType=1, tPrecision=64, zPrecision=10)\n"
+            "assert isinstance(m, _NonlinearMedium) and m.omega.shape == (64,)\n"
+            "assert nonlinearmedium.Chi2SHG.__mro__[1] is _FullyNonlinearMedium\n"
+            "print('ok')\n" % REPO)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120, cwd="/")
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr[-2000:]

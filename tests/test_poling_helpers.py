@@ -56,3 +56,29 @@ def test_bloch_messiah_shape_errors():
         bloch_messiah(np.eye(5))
     om = sympmat(2)
     assert np.array_equal(om @ om, -np.eye(4)) and np.array_equal(om.T, -om)
+
+
+def test_linear_poling_hand_case():
+    # L = 1, dL = 0.25: samples z = 0, 1/3, 2/3, 1 of sin(6 z) -> 0 (counts +1), +, -, -
+    np.testing.assert_allclose(P.linearPoling(6.0, 6.0, 1.0, 0.25), [0.5, 0.5])
+    # a chirp: phi = (kf - k0) z^2 / (2 L) + k0 z with k0 = 0, kf = 12, L = 1 is 6 z^2;
+    # samples z = 0, .25, .5, .75, 1 (dL = 0.2): sin 0, sin .375, sin 1.5, sin 3.375, sin 6 -> + + + - -
+    np.testing.assert_allclose(P.linearPoling(0.0, 12.0, 1.0, 0.2), [0.6, 0.4])
+
+
+def test_linear_poling_constant_rate_is_periodic():
+    k = math.pi  # half period 1
+    d = P.linearPoling(k, k, 10.0, 1e-3)
+    assert abs(d.sum() - 10.0) < 1e-9
+    assert np.all(np.abs(d[1:-1] - math.pi / k) <= 2e-3)
+
+
+def test_linear_poling_domains_follow_the_local_wavenumber():
+    k0, kf, L = 4.0, 40.0, 5.0
+    d = P.linearPoling(k0, kf, L, 1e-4)
+    ends = np.cumsum(d)
+    mids = ends - d / 2
+    local = k0 + (kf - k0) * mids / L
+    # each interior domain spans half a local period, pi / k(z), to the sampling resolution
+    rel = np.abs(d[1:-1] * local[1:-1] / math.pi - 1)
+    assert rel.max() < 0.02
