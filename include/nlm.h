@@ -30,6 +30,7 @@
 #ifndef NLM_H
 #define NLM_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -150,6 +151,21 @@ int nlm_batch(nlm_plan* plan, const double* inputs, int rows, int cols, int in_t
 int nlm_greens(nlm_plan* plan, int in_time_domain, int n_in, const int32_t* inputs, int n_out,
                const int32_t* outputs, const double* block_scale, double* C, double* S);
 int nlm_greens_download(nlm_plan* plan, double* C, double* S);
+
+/* No counterpart in the reference: nlm_greens for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  It calls no hipMalloc, hipFree, memset or other allocating
+ * routine: the Green's rows ([2 N n_in][N n_out] complex) and the step-propagator
+ * table of one-mode undepleted media ([nZSteps][N][2] complex) live in a fixed
+ * device array of NLM_NOALLOC_SCRATCH_BYTES that each GPU holds from its first
+ * call until the process ends; larger requests fail with NLM_INVALID_ARGUMENT.
+ * C_dev, S_dev: caller-owned device buffers of (n_out*N) x (n_in*N) complex.
+ * No block normalisation.  Asynchronous on the plan's stream (nlm_plan_sync).
+ * The pump grid must have been computed.  All calls on one GPU share its array:
+ * calls that overlap on one GPU race. */
+#define NLM_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int nlm_greens_noalloc(nlm_plan* plan, int in_time_domain, int n_in, const int32_t* inputs, int n_out,
+                       const int32_t* outputs, double* C_dev, double* S_dev);
 
 /* Device-resident variants for callers that keep data in HBM (e.g. PyTorch
  * tensors): same semantics as nlm_batch with device pointers. */

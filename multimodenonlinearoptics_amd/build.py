@@ -80,6 +80,11 @@ def build(force=False, jobs=None, verbose=False):
     deps = _deps("nlm_compose.hip", "nlm_internal.hpp", *dev_hdrs)
     if force or _stale(o, deps):
         steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_compose.hip"), "-o", o])
+    o = os.path.join(OBJ, "nlm_noalloc.o")
+    objs.append(o)
+    deps = _deps("nlm_noalloc.hip", "nlm_internal.hpp", *dev_hdrs)
+    if force or _stale(o, deps):
+        steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_noalloc.hip"), "-o", o])
     o = os.path.join(OBJ, "nlm_prop_tab.o")
     objs.append(o)
     deps = _deps("nlm_prop_tab.hip", *dev_hdrs)

@@ -19,7 +19,7 @@ SYMBOLS = [
     "nlm_multi_create", "nlm_multi_launch", "nlm_multi_sync", "nlm_multi_output", "nlm_multi_classes",
     "nlm_multi_grid", "nlm_multi_destroy", "nlm_batch_forward", "nlm_batch_adjoint",
     "nlm_greens_compose_dev", "nlm_greens_cascade", "nlm_pump_import",
-    "nlm_batch_grids", "nlm_gaussian_covariance", "nlm_gram", "nlm_gemm_nt",
+    "nlm_batch_grids", "nlm_gaussian_covariance", "nlm_gram", "nlm_gemm_nt", "nlm_greens_noalloc",
 ]
 
 _lib = None
@@ -55,6 +55,7 @@ def lib():
         L.nlm_pump_linear.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp]
         L.nlm_greens.argtypes = [vp, C.c_int, C.c_int, ip, C.c_int, ip, vp, vp, vp]
         L.nlm_greens_download.argtypes = [vp, vp, vp]
+        L.nlm_greens_noalloc.argtypes = [vp, C.c_int, C.c_int, ip, C.c_int, ip, vp, vp]
         L.nlm_batch.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, vp]
         L.nlm_batch_dev.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, vp]
         L.nlm_greens_compose.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]

@@ -704,16 +704,28 @@ void input_channels(int nS, int nCh, int inputMode, int* inPos) {
   }
 }
 
+bool uses_step_table(const nlm_plan* p) {
+  return p->stepTable && (p->model == NLM_CHI2PDC || p->model == NLM_CHI3);
+}
+
 int launch_prop_kernel(nlm_plan* p, const KParams& kp0) {
+  cplx* tab = nullptr;
+  if (uses_step_table(p)) {
+    if (int e = p->stepTab.ensure((size_t)p->nZ * p->N * 2)) return e;
+    tab = p->stepTab.p;
+  }
+  return launch_prop_kernel_tab(p, kp0, tab);
+}
+
+int launch_prop_kernel_tab(nlm_plan* p, const KParams& kp0, cplx* tab) {
   if (p->L > 13)
     return set_err(NLM_INVALID_ARGUMENT, "nFreqs above 8192 is available for Chi3GNLSE batchSignalSimulation only");
   KParams kp = kp0;
   prop_fn fn = nullptr;
-  if (p->stepTable && (p->model == NLM_CHI2PDC || p->model == NLM_CHI3)) {
-    if (int e = p->stepTab.ensure((size_t)p->nZ * p->N * 2)) return e;
-    nlm_launch_step_table(p->model, p->coef.p, p->cconst.p, p->pumpT[0].p, p->N, p->nZ, p->stepTab.p, p->stream);
+  if (uses_step_table(p)) {
+    nlm_launch_step_table(p->model, p->coef.p, p->cconst.p, p->pumpT[0].p, p->N, p->nZ, tab, p->stream);
     HIP_TRY(hipGetLastError());
-    kp.tab = p->stepTab.p;
+    kp.tab = tab;
     fn = nlm_pick_prop_tab(p->L);
   } else {
     fn = nlm_pick_prop(p->model, p->L);

@@ -90,6 +90,10 @@ inline void plan_touch(nlm_plan_s* p) { ++p->gen; }
 void fill_common(nlm_plan_s* p, KParams& kp);
 int check_pumps_ready(nlm_plan_s* p);
 int launch_prop_kernel(nlm_plan_s* p, const KParams& kp);
+// the same with the step-propagator table [nZ][N][2] in a caller's buffer (used when
+// uses_step_table(p)); allocates nothing
+int launch_prop_kernel_tab(nlm_plan_s* p, const KParams& kp, cplx* tab);
+bool uses_step_table(const nlm_plan_s* p);
 int get_twiddles(int n, const cplx** out);
 // input channel feeding each mode (-1: none), src/_NonlinearMedium.hpp:164-198
 void input_channels(int nS, int nCh, int inputMode, int* inPos);

@@ -303,6 +303,9 @@ PYBIND11_MODULE(nonlinearmedium, m) {
            },
            "dL/d(inputProfs) of the last batchForward for dL/d(outputs) (convention dL/dRe + i dL/dIm; "
            "extension; Chi3GNLSE).", "gOut"_a);
+  base.def("planHandle", [](Medium& md) { return reinterpret_cast<std::uintptr_t>(md.plan()); },
+           "The medium's nlm_plan* (include/nlm.h) as an integer, with its tables current, for callers of "
+           "the C-ABI (extension).");
   base.def_property_readonly("forwardSerial", &Medium::forwardSerial,
                              "Number of greensForward / batchForward calls made on this medium (extension): "
                              "an adjoint belongs to the forward whose serial is current.");
