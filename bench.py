@@ -115,7 +115,8 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4):
     def rel(a, b):
         return abs(a - b) / max(abs(b), 1e-300)
 
-    fwd = forward_mode_reference(config4_parts(N, nZ)[0], spec0, phi, pol, dirs[:fwd_mode_dirs])
+    fwd = forward_mode_reference(config4_parts(N, nZ)[0], spec0, phi, pol, dirs[:fwd_mode_dirs]) \
+        if fwd_mode_dirs > 0 else None
     out = {
         "workload": "Chi2PDC N=1024 nZ=2000, loss ||S S^T - 1.5 I||_F^2, grad w.r.t. pump spectral phase "
                     "(1024) and per-step poling (2000)",
@@ -487,6 +488,10 @@ def main():
                     help="nZSteps of the Chi3GNLSE section (the medium is shortened in proportion)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads for the CPU baseline (default: every core this process may use, at most 16)")
+    ap.add_argument("--grad-dirs", type=int, default=32,
+                    help="seeded directions of the config-4 finite-difference check")
+    ap.add_argument("--fwd-mode-dirs", type=int, default=4,
+                    help="directions of the config-4 forward-mode reference (0: skip)")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="only run a device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -536,7 +541,7 @@ def main():
         elif args.only == "compose":
             out = compose_benchmark()
         else:
-            out = grad_benchmark(nlm)
+            out = grad_benchmark(nlm, n_dirs=args.grad_dirs, fwd_mode_dirs=args.fwd_mode_dirs)
         print(json.dumps(out))
         return
     medium = config2(nlm, N_FREQS, NZ)
@@ -629,7 +634,7 @@ def main():
         rec["compose"] = compose_benchmark()
 
     if not args.no_grad:
-        g = grad_benchmark(nlm)
+        g = grad_benchmark(nlm, n_dirs=args.grad_dirs, fwd_mode_dirs=args.fwd_mode_dirs)
         rec["grad"] = g
         rec["grad_vs_forward"] = g["grad_vs_forward"]
 

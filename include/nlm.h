@@ -284,6 +284,12 @@ int nlm_gemm_nt(int M, int Ncol, int K, const double* A, const double* B, double
  * count rows of length n, forward unscaled / inverse scaled by 1/n. */
 int nlm_fft(int n, int count, const double* in, double* out, int inverse);
 
+/* Extension: the FFT schedule variant (0..3, see csrc/nlm_fft.hpp FftShape) the
+ * autotuner chose on the current device for step-table propagations of n_freqs
+ * points, kind (0 Green's function, 1 batch, 2 z-resolved grid) and a field count
+ * of n_fields (rounded up to a power of two); -1 if none has been tuned. */
+int nlm_fft_schedule(int n_freqs, int kind, int n_fields);
+
 /* Synchronise the plan's stream. */
 int nlm_plan_sync(nlm_plan* plan);
 

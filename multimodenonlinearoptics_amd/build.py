@@ -85,11 +85,13 @@ def build(force=False, jobs=None, verbose=False):
     deps = _deps("nlm_noalloc.hip", "nlm_internal.hpp", *dev_hdrs)
     if force or _stale(o, deps):
         steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_noalloc.hip"), "-o", o])
-    o = os.path.join(OBJ, "nlm_prop_tab.o")
-    objs.append(o)
-    deps = _deps("nlm_prop_tab.hip", *dev_hdrs)
-    if force or _stale(o, deps):
-        steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_prop_tab.hip"), "-o", o])
+    for v in range(4):  # FFT schedule variants of the step-table kernel (chosen by the autotuner)
+        o = os.path.join(OBJ, "nlm_prop_tab_v%d.o" % v)
+        objs.append(o)
+        deps = _deps("nlm_prop_tab.hip", *dev_hdrs)
+        if force or _stale(o, deps):
+            steps.append([HIPCC] + HIP_FLAGS + ["-DNLM_TAB_VARIANT=%d" % v, "-c",
+                                                os.path.join(CSRC, "nlm_prop_tab.hip"), "-o", o])
     for m in PROP_MODELS:
         o = os.path.join(OBJ, "nlm_prop_%d.o" % m)
         objs.append(o)

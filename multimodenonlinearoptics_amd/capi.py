@@ -20,6 +20,7 @@ SYMBOLS = [
     "nlm_multi_grid", "nlm_multi_destroy", "nlm_batch_forward", "nlm_batch_adjoint",
     "nlm_greens_compose_dev", "nlm_greens_cascade", "nlm_pump_import",
     "nlm_batch_grids", "nlm_gaussian_covariance", "nlm_gram", "nlm_gemm_nt", "nlm_greens_noalloc",
+    "nlm_fft_schedule",
 ]
 
 _lib = None

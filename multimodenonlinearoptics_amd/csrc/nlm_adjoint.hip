@@ -53,6 +53,10 @@ static void launch_env(const cplx* gP, int nZP, const double* Dp, double dzp, co
 static sweep_fn pick_sweep(int L) { NLM_L_SWITCH2(launch_sweep, L) }
 static env_fn pick_env(int L) { NLM_L_SWITCH2(launch_env, L) }
 
+// The sweep keeps the default FFT schedule: with the forward's schedule variants
+// (see FftShape) forced on it, the backward at N = 1024, nZ = 2000 took 81.6,
+// 91.4 and 85.5 ms against 77.9 ms.
+
 static int sweep_fields_per_wg(int L) {
   switch (L) {
     case 3: return AdjShape<3>::FA;

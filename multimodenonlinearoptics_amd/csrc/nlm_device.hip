@@ -714,29 +714,107 @@ int launch_prop_kernel(nlm_plan* p, const KParams& kp0) {
     if (int e = p->stepTab.ensure((size_t)p->nZ * p->N * 2)) return e;
     tab = p->stepTab.p;
   }
-  return launch_prop_kernel_tab(p, kp0, tab);
+  return launch_prop_kernel_tab(p, kp0, tab, true);
 }
 
-int launch_prop_kernel_tab(nlm_plan* p, const KParams& kp0, cplx* tab) {
+// FFT schedule autotuner of the step-table propagation (nFreqs 512..8192 with at
+// least 128 fields): the first launch of a (device, nFreqs, kind, field-count
+// bucket) runs every schedule variant of the kernel on the real inputs (the
+// launch is idempotent: same inputs, same outputs overwritten), keeps the fastest
+// by HIP events, and runs it once more so that the call's outputs are that
+// variant's.  Later launches with the same key use the stored choice.
+// NLM_AUTOTUNE=0 keeps the default schedule; NLM_FFT_SCHEDULE=v forces variant v.
+namespace {
+std::mutex g_tune_mu;
+std::map<long long, int> g_tuned;
+hipEvent_t g_tune_ev[2] = {nullptr, nullptr};
+
+int env_int(const char* name, int dflt) {
+  const char* e = getenv(name);
+  return e && *e ? atoi(e) : dflt;
+}
+
+long long tune_key(const nlm_plan* p, const KParams& kp) {
+  int bucket = 0;
+  while ((1 << bucket) < kp.nFields) bucket++;
+  return (((long long)p->device * 64 + p->L) * 16 + kp.kind) * 64 + bucket;
+}
+
+int tune_variant(nlm_plan* p, const KParams& kp, int grid, int* best) {
+  if (!g_tune_ev[0]) {
+    HIP_TRY(hipEventCreate(&g_tune_ev[0]));
+    HIP_TRY(hipEventCreate(&g_tune_ev[1]));
+  }
+  float best_ms = 1e30f;
+  *best = 0;
+  for (int pass = 0; pass < 2; pass++)  // the first pass warms the clocks and the code objects
+    for (int v = 0; v < 4; v++) {
+      prop_fn f = nlm_pick_prop_tab_variant(p->L, v);
+      if (!f) continue;
+      HIP_TRY(hipEventRecord(g_tune_ev[0], p->stream));
+      f(kp, grid, p->stream);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipEventRecord(g_tune_ev[1], p->stream));
+      HIP_TRY(hipEventSynchronize(g_tune_ev[1]));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, g_tune_ev[0], g_tune_ev[1]));
+      if (pass == 1 && ms < best_ms) {
+        best_ms = ms;
+        *best = v;
+      }
+    }
+  return NLM_OK;
+}
+}  // namespace
+
+int launch_prop_kernel_tab(nlm_plan* p, const KParams& kp0, cplx* tab, bool may_tune) {
   if (p->L > 13)
     return set_err(NLM_INVALID_ARGUMENT, "nFreqs above 8192 is available for Chi3GNLSE batchSignalSimulation only");
   KParams kp = kp0;
   prop_fn fn = nullptr;
+  int fpb = 1, T = p->N / 8;
+  if (T < 64) fpb = 64 / T;
+  const int grid = (kp.nFields + fpb - 1) / fpb;
   if (uses_step_table(p)) {
     nlm_launch_step_table(p->model, p->coef.p, p->cconst.p, p->pumpT[0].p, p->N, p->nZ, tab, p->stream);
     HIP_TRY(hipGetLastError());
     kp.tab = tab;
     fn = nlm_pick_prop_tab(p->L);
+    const int forced = env_int("NLM_FFT_SCHEDULE", -1);
+    if (forced >= 0) {
+      if (prop_fn f = nlm_pick_prop_tab_variant(p->L, forced)) fn = f;
+    } else if (p->L >= 9 && kp.nFields >= 128 && env_int("NLM_AUTOTUNE", 1) != 0) {
+      std::lock_guard<std::mutex> lock(g_tune_mu);
+      const long long key = tune_key(p, kp);
+      auto it = g_tuned.find(key);
+      int v = 0;
+      if (it != g_tuned.end()) {
+        v = it->second;
+      } else if (may_tune) {
+        if (int e = tune_variant(p, kp, grid, &v)) return e;
+        g_tuned[key] = v;
+      }
+      if (prop_fn f = nlm_pick_prop_tab_variant(p->L, v)) fn = f;
+    }
   } else {
     fn = nlm_pick_prop(p->model, p->L);
   }
   if (!fn) return set_err(NLM_INVALID_ARGUMENT, "unsupported model / size");
-  int fpb = 1, T = p->N / 8;
-  if (T < 64) fpb = 64 / T;
-  const int grid = (kp.nFields + fpb - 1) / fpb;
   fn(kp, grid, p->stream);
   HIP_TRY(hipGetLastError());
   return NLM_OK;
+}
+
+extern "C" int nlm_fft_schedule(int n_freqs, int kind, int n_fields) {
+  int L = 0;
+  while ((1 << L) < n_freqs) L++;
+  int bucket = 0;
+  while ((1 << bucket) < n_fields) bucket++;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tuned.find((((long long)dev * 64 + L) * 16 + kind) * 64 + bucket);
+  return it == g_tuned.end() ? -1 : it->second;
 }
 
 extern "C" int nlm_plan_set_step_table(nlm_plan* p, int enable) {

@@ -53,12 +53,22 @@ __device__ __forceinline__ cplx mul_mi(cplx a) {
   return INV ? cplx{-a.im, a.re} : cplx{a.im, -a.re};
 }
 
-template <int L>
+// A transform is described by a shape code C = log2(N) + 32 * V.  V selects a
+// schedule variant, picked by the autotuner of the propagation launch (V = 0 is
+// the default schedule):
+//   bit 0: the other LDS padding (one slot per 8 elements where the default is
+//          one per 32, and the reverse);
+//   bit 1: the stage of radix 2 or 4 (log2 N not a multiple of 3) goes last
+//          instead of second.
+template <int C>
 struct FftShape {
-  static constexpr int N = 1 << L;
-  static constexpr int T = N / 8;                       // lanes per field
-  static constexpr int S = L / 3 + ((L % 3) ? 1 : 0);   // number of stages
-  static constexpr int radix(int s) { return (s == 1 && (L % 3)) ? (1 << (L % 3)) : 8; }
+  static constexpr int LOG = C & 31;
+  static constexpr int V = C >> 5;
+  static constexpr int N = 1 << LOG;
+  static constexpr int T = N / 8;                             // lanes per field
+  static constexpr int S = LOG / 3 + ((LOG % 3) ? 1 : 0);     // number of stages
+  static constexpr int SMALL = (V & 2) ? S - 1 : 1;           // stage of the small radix
+  static constexpr int radix(int s) { return (s == SMALL && (LOG % 3)) ? (1 << (LOG % 3)) : 8; }
   static constexpr int before(int s) {                  // product of radices of stages < s
     int p = 1;
     for (int i = 0; i < s; i++) p *= radix(i);
@@ -67,7 +77,8 @@ struct FftShape {
   // LDS padding: one slot after every 2^PAD elements.  Measured on the step
   // kernels (N = 512: 5.6 % faster with PAD 5 than 3; N = 1024: the adjoint
   // sweep is 4 % slower with 5, PAD 2 and 4 slower everywhere).
-  static constexpr int PAD = L <= 9 ? 5 : 3;
+  static constexpr int PAD_DEFAULT = LOG <= 9 ? 5 : 3;
+  static constexpr int PAD = (V & 1) ? (PAD_DEFAULT == 5 ? 3 : 5) : PAD_DEFAULT;
   static constexpr int lds_elems = N + (N >> PAD);       // padded LDS footprint (cplx)
 };
 
@@ -213,9 +224,9 @@ struct TableTwiddles {
 // the same from N = 2048 was 1 % slower.)
 template <int L>
 struct PropTw {
-  static constexpr bool LEAN = L >= 12;
-  static constexpr bool IN_LDS = L == 12;
-  static constexpr int LDS_ELEMS = IN_LDS ? (1 << L) : 1;
+  static constexpr bool LEAN = FftShape<L>::LOG >= 12;
+  static constexpr bool IN_LDS = FftShape<L>::LOG == 12;
+  static constexpr int LDS_ELEMS = IN_LDS ? FftShape<L>::N : 1;
   using type = typename std::conditional<LEAN, TableTwiddles<L>, RegTwiddles<L>>::type;
 };
 
@@ -223,7 +234,7 @@ struct PropTw {
 template <int L>
 __device__ __forceinline__ void fill_twiddle_lds(cplx* lds_tab, const cplx* table) {
   if constexpr (PropTw<L>::IN_LDS) {
-    for (int k = threadIdx.x; k < (1 << L); k += blockDim.x) lds_tab[k] = table[k];
+    for (int k = threadIdx.x; k < FftShape<L>::N; k += blockDim.x) lds_tab[k] = table[k];
     __syncthreads();
   }
 }

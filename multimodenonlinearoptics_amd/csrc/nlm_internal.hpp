@@ -92,7 +92,7 @@ int check_pumps_ready(nlm_plan_s* p);
 int launch_prop_kernel(nlm_plan_s* p, const KParams& kp);
 // the same with the step-propagator table [nZ][N][2] in a caller's buffer (used when
 // uses_step_table(p)); allocates nothing
-int launch_prop_kernel_tab(nlm_plan_s* p, const KParams& kp, cplx* tab);
+int launch_prop_kernel_tab(nlm_plan_s* p, const KParams& kp, cplx* tab, bool may_tune = false);
 bool uses_step_table(const nlm_plan_s* p);
 int get_twiddles(int n, const cplx** out);
 // input channel feeding each mode (-1: none), src/_NonlinearMedium.hpp:164-198

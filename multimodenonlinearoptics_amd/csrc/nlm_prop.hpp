@@ -484,5 +484,8 @@ static void launch_gnlse(const KParams& kp, int grid, hipStream_t s) {
 prop_fn nlm_pick_prop(int model, int L);
 // Step-propagator kernel (one signal mode) for log2 N = L, and its table builder.
 prop_fn nlm_pick_prop_tab(int L);
+// schedule variant 0..3 of the table kernel (see FftShape); nullptr where a
+// variant does not exist or equals the default
+prop_fn nlm_pick_prop_tab_variant(int L, int variant);
 bool nlm_launch_step_table(int model, const cplx* coef, const cplx* cconst, const cplx* pump, int N, int nZ,
                            cplx* tab, hipStream_t s);

@@ -136,8 +136,20 @@ static void launch_prop_tab(const KParams& kp, int grid, hipStream_t s) {
   hipLaunchKernelGGL((k_prop_tab<L>), dim3(grid), dim3(Launch<L>::THREADS), 0, s, kp);
 }
 
-prop_fn nlm_pick_prop_tab(int L) {
+// This file is compiled once per schedule variant NLM_TAB_VARIANT (see FftShape);
+// variants other than 0 exist for nFreqs 512..8192, where the autotuner chooses.
+#ifndef NLM_TAB_VARIANT
+#define NLM_TAB_VARIANT 0
+#endif
+#define NLM_CAT2(a, b) a##b
+#define NLM_CAT(a, b) NLM_CAT2(a, b)
+
+prop_fn NLM_CAT(nlm_pick_prop_tab_var, NLM_TAB_VARIANT)(int L) {
+  constexpr int V = NLM_TAB_VARIANT, C = 32 * V;
+  // bit 1 (small radix last) only differs from the default when log2 N is not a multiple of 3
+  if (V != 0 && (L < 9 || L > 13 || ((V & 2) && L % 3 == 0))) return nullptr;
   switch (L) {
+#if NLM_TAB_VARIANT == 0
     case 3: return &launch_prop_tab<3>;
     case 4: return &launch_prop_tab<4>;
     case 5: return &launch_prop_tab<5>;
@@ -145,10 +157,31 @@ prop_fn nlm_pick_prop_tab(int L) {
     case 7: return &launch_prop_tab<7>;
     case 8: return &launch_prop_tab<8>;
     case 9: return &launch_prop_tab<9>;
-    case 10: return &launch_prop_tab<10>;
-    case 11: return &launch_prop_tab<11>;
     case 12: return &launch_prop_tab<12>;
-    case 13: return &launch_prop_tab<13>;
+#elif (NLM_TAB_VARIANT & 2) == 0
+    case 9: return &launch_prop_tab<9 + C>;
+    case 12: return &launch_prop_tab<12 + C>;
+#endif
+    case 10: return &launch_prop_tab<10 + C>;
+    case 11: return &launch_prop_tab<11 + C>;
+    case 13: return &launch_prop_tab<13 + C>;
+  }
+  return nullptr;
+}
+
+#if NLM_TAB_VARIANT == 0
+prop_fn nlm_pick_prop_tab_var1(int L);
+prop_fn nlm_pick_prop_tab_var2(int L);
+prop_fn nlm_pick_prop_tab_var3(int L);
+
+prop_fn nlm_pick_prop_tab(int L) { return nlm_pick_prop_tab_var0(L); }
+
+prop_fn nlm_pick_prop_tab_variant(int L, int variant) {
+  switch (variant) {
+    case 0: return nlm_pick_prop_tab_var0(L);
+    case 1: return nlm_pick_prop_tab_var1(L);
+    case 2: return nlm_pick_prop_tab_var2(L);
+    case 3: return nlm_pick_prop_tab_var3(L);
   }
   return nullptr;
 }
@@ -169,3 +202,4 @@ bool nlm_launch_step_table(int model, const cplx* coef, const cplx* cconst, cons
     return false;
   return true;
 }
+#endif  // NLM_TAB_VARIANT == 0
