@@ -58,7 +58,7 @@ def flops_per_field_step(n):
     return 10 * n * math.log2(n) + 80 * n
 
 
-def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4):
+def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4, ckpt_every=0):
     """Differentiable Chi2PDC (N=1024, nZ=2000): loss ||S S^T - 1.5 I||_F^2 of the
     Green's function S, gradient w.r.t. the pump spectral phase (1024 reals) and the
     per-step poling (2000 reals) by the adjoint kernel with ~sqrt(nZ) checkpoints.
@@ -91,7 +91,7 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4):
     t_ck, t_bwd, k_bwd = [], [], []
     for _ in range(reps):
         t0 = time.perf_counter()
-        C, S = m.greensForward(False, True, 0)
+        C, S = m.greensForward(False, True, ckpt_every)
         t_ck.append(time.perf_counter() - t0)
         M = S @ S.T - 1.5 * np.eye(N)
         gS = 4.0 * (M @ np.conj(S))
@@ -100,7 +100,7 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4):
         t_bwd.append(time.perf_counter() - t0)
         tm = m.lastTiming()
         k_bwd.append(tm[1] + tm[2])
-    L0, g_phi, g_pol = config4_adjoint(m, spec0, phi, pol)
+    L0, g_phi, g_pol = config4_adjoint(m, spec0, phi, pol, ckpt_every=ckpt_every)
     dirs = config4_directions(n_dirs, N, nZ)
     ads = [float(np.dot(g_phi, dp) + np.dot(g_pol, dq)) for dp, dq in dirs]
     raw, rich = [], []
@@ -392,8 +392,8 @@ def compose_benchmark(n=512, reps=20):
 
 def host_cpu():
     """The host's CPU model and count, and the cores this process may use: its
-    affinity mask, capped by a cgroup CPU quota and by the 16-core share a GPU box
-    gives each GPU."""
+    affinity mask, capped by a cgroup CPU quota and by 16 cores (the host share of
+    one GPU on a shared multi-GPU node)."""
     model = None
     try:
         with open("/proc/cpuinfo") as f:
@@ -490,6 +490,8 @@ def main():
                     help="threads for the CPU baseline (default: every core this process may use, at most 16)")
     ap.add_argument("--grad-dirs", type=int, default=32,
                     help="seeded directions of the config-4 finite-difference check")
+    ap.add_argument("--ckpt-every", type=int, default=0,
+                    help="checkpoint interval of the config-4 forward (0: the engine's choice)")
     ap.add_argument("--fwd-mode-dirs", type=int, default=4,
                     help="directions of the config-4 forward-mode reference (0: skip)")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
@@ -541,7 +543,8 @@ def main():
         elif args.only == "compose":
             out = compose_benchmark()
         else:
-            out = grad_benchmark(nlm, n_dirs=args.grad_dirs, fwd_mode_dirs=args.fwd_mode_dirs)
+            out = grad_benchmark(nlm, n_dirs=args.grad_dirs, fwd_mode_dirs=args.fwd_mode_dirs,
+                                 ckpt_every=args.ckpt_every)
         print(json.dumps(out))
         return
     medium = config2(nlm, N_FREQS, NZ)
@@ -634,7 +637,8 @@ def main():
         rec["compose"] = compose_benchmark()
 
     if not args.no_grad:
-        g = grad_benchmark(nlm, n_dirs=args.grad_dirs, fwd_mode_dirs=args.fwd_mode_dirs)
+        g = grad_benchmark(nlm, n_dirs=args.grad_dirs, fwd_mode_dirs=args.fwd_mode_dirs,
+                           ckpt_every=args.ckpt_every)
         rec["grad"] = g
         rec["grad_vs_forward"] = g["grad_vs_forward"]
 
