@@ -23,7 +23,10 @@
  *    the reference's std::invalid_argument, NLM_RUNTIME_ERROR to
  *    std::runtime_error;
  *  - transforms follow src/CustomEigenFFT.h:54-137: forward unscaled,
- *    inverse scaled by 1/n.  The device path supports n = 2^k, 8 <= n <= 8192.
+ *    inverse scaled by 1/n.  Any even n is supported: n = 2^k, 8 <= n <= 8192
+ *    runs the in-register kernels, every other even n (and Chi3GNLSE batches
+ *    of 16384 and 32768) the mixed-radix and staged paths; Chi3GNLSE and the
+ *    adjoints need powers of two.
  *  - a plan is not safe for concurrent calls; distinct plans may be used from
  *    distinct threads.
  */
@@ -219,15 +222,17 @@ int nlm_plan_last_timing(nlm_plan* plan, double ms[3]);
  * the pump, conjugate-coupled signal; computeGreensFunction,
  * src/_NonlinearMedium.cpp:263-375, with Chi2PDC::DiffEq,
  * solvers/Chi2PDC.cpp:27-47).  nlm_greens_forward is nlm_greens with
- * time-domain field checkpoints every `ckpt_every` z steps (<= 0: ceil(sqrt(nZ-1)))
- * kept on the device.  nlm_greens_adjoint then takes dL/dC, dL/dS (complex,
+ * time-domain field checkpoints every `ckpt_every` z steps kept on the device
+ * (<= 0: the smallest interval whose 2N x ceil((nZ-1)/K) x N x 16 bytes fit in a
+ * quarter of the device memory and half of the free memory, at most
+ * ceil(sqrt(nZ-1))).  nlm_greens_adjoint then takes dL/dC, dL/dS (complex,
  * convention dL/dRe + i dL/dIm, the layout of C and S) and returns
  *   g_coef          [nZ][3][n_step_coef] gradient w.r.t. the step coefficients,
  *   g_pump_time     [2nZ-1][N] gradient w.r.t. the pump time grid (or NULL),
  *   g_envelope      [N] gradient w.r.t. the time-domain pump envelope through
  *                   the linear pump grid (nlm_pump_linear must have built it; or NULL).
- * The reverse sweep recomputes each checkpoint segment; reductions are done in
- * a fixed order (no atomics). */
+ * The reverse sweep rebuilds the states between checkpoints; reductions are
+ * done in a fixed order (no atomics). */
 int nlm_greens_forward(nlm_plan* plan, int in_time_domain, int n_in, const int32_t* inputs, int n_out,
                        const int32_t* outputs, const double* block_scale, int ckpt_every, double* C, double* S);
 int nlm_greens_adjoint(nlm_plan* plan, const double* gC, const double* gS, double* g_coef, double* g_pump_time,

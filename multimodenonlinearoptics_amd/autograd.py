@@ -4,7 +4,8 @@
 matrices (C, S) of a single-mode Chi2PDC medium as complex128 tensors, for a
 time-domain pump envelope (complex, length nFreqs) and a per-z-step poling
 profile (real, length nZSteps).  Its backward runs the device adjoint (forward
-checkpoints every ~sqrt(nZ) steps, recomputation between them) and returns
+checkpoints of the field, every step when they fit in a quarter of the device
+memory, else every few steps with the states between them rebuilt) and returns
 dL/d envelope and dL/d poling in torch's complex-gradient convention.
 
 The forward replaces computeGreensFunction (src/_NonlinearMedium.cpp:263-375)

@@ -80,6 +80,11 @@ def build(force=False, jobs=None, verbose=False):
     deps = _deps("nlm_compose.hip", "nlm_internal.hpp", *dev_hdrs)
     if force or _stale(o, deps):
         steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_compose.hip"), "-o", o])
+    o = os.path.join(OBJ, "nlm_any.o")
+    objs.append(o)
+    deps = _deps("nlm_any.hip", "nlm_internal.hpp", *dev_hdrs)
+    if force or _stale(o, deps):
+        steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_any.hip"), "-o", o])
     o = os.path.join(OBJ, "nlm_noalloc.o")
     objs.append(o)
     deps = _deps("nlm_noalloc.hip", "nlm_internal.hpp", *dev_hdrs)

@@ -361,6 +361,7 @@ static int log2_exact(int n) {
 }
 
 static int check_n(int n, int* L) {
+  if (n <= 0 || n % 2) return set_err(NLM_INVALID_ARGUMENT, "Invalid number of Frequencies");
   *L = log2_exact(n);
   if (*L < LMIN || *L > LMAX)
     return set_err(NLM_INVALID_ARGUMENT, "nFreqs must be a power of two between 8 and 8192 on the MI355X path (got " +
@@ -434,6 +435,8 @@ extern "C" int nlm_plan_create(const nlm_plan_desc* d, nlm_plan** out) {
   int L;
   if (d->model == NLM_CHI3GNLSE && (d->n_freqs == 16384 || d->n_freqs == 32768)) {
     L = log2_exact(d->n_freqs);  // staged transform (batches only)
+  } else if (d->model != NLM_CHI3GNLSE && nlm_any_size(d->n_freqs)) {
+    L = 0;  // any-length path (nlm_any.hip)
   } else if (int e = check_n(d->n_freqs, &L)) {
     return e;
   }
@@ -571,10 +574,14 @@ extern "C" int nlm_pump_linear(nlm_plan* p, int n_pump, const double* envelope, 
   p->hasStrength = strength != nullptr;
   p->dzp = dzp;
   tic(p, 0);
-  pick_fft(p->L)(p->env.p, p->F0.p, p->nP, p->tw, false, p->stream);
-  for (int q = 0; q < p->nP; q++)
-    pick_pump_linear(p->L)(p->F0.p + (size_t)q * N, p->env.p + (size_t)q * N, p->Dp.p + (size_t)q * N, dzp,
-                           strength ? p->strength.p : nullptr, nZP, p->pumpF[q].p, p->pumpT[q].p, p->tw, p->stream);
+  if (p->L == 0) {
+    if (int e = nlm_any_pump_linear(p, strength ? p->strength.p : nullptr, dzp)) return e;
+  } else {
+    pick_fft(p->L)(p->env.p, p->F0.p, p->nP, p->tw, false, p->stream);
+    for (int q = 0; q < p->nP; q++)
+      pick_pump_linear(p->L)(p->F0.p + (size_t)q * N, p->env.p + (size_t)q * N, p->Dp.p + (size_t)q * N, dzp,
+                             strength ? p->strength.p : nullptr, nZP, p->pumpF[q].p, p->pumpT[q].p, p->tw, p->stream);
+  }
   HIP_TRY(hipGetLastError());
   tic(p, 1);
   if (p->timing) collect_timing(p, 0, 1, 0);
@@ -596,8 +603,12 @@ extern "C" int nlm_pump_chi3(nlm_plan* p, const double* envelope, const double* 
   if (int e = upload(p->relI, rel_intensity, nZP, p->stream)) return e;
   p->pumpLinear = false;
   tic(p, 0);
-  pick_pump_chi3(p->L)(p->env.p, p->Dp.p, dzp, p->dsp.p, p->relI.p, nZP, p->pumpF[0].p, p->pumpT[0].p, p->tw,
-                       p->stream);
+  if (p->L == 0) {
+    if (int e = nlm_any_pump_chi3(p, dzp)) return e;
+  } else {
+    pick_pump_chi3(p->L)(p->env.p, p->Dp.p, dzp, p->dsp.p, p->relI.p, nZP, p->pumpF[0].p, p->pumpT[0].p, p->tw,
+                         p->stream);
+  }
   HIP_TRY(hipGetLastError());
   tic(p, 1);
   if (p->timing) collect_timing(p, 0, 1, 0);
@@ -647,10 +658,16 @@ extern "C" int nlm_pump_import(nlm_plan* p, int q, nlm_plan* src, int mode, cons
   HIP_TRY(hipMemcpyAsync(tab.p + N, disp_signal_src, N * sizeof(double), hipMemcpyHostToDevice, p->stream));
   HIP_TRY(hipMemcpyAsync(tab.p + 2 * N, omega, N * sizeof(double), hipMemcpyHostToDevice, p->stream));
   HIP_TRY(hipStreamSynchronize(src->stream));  // the source grid is complete
-  const import_fn launch = pick_import(p->L);
-  if (!launch) return set_err(NLM_INVALID_ARGUMENT, "unsupported size for the device pump import");
-  launch(src->gridF.p + (size_t)mode * src->nZ * N, src->nZ, dz_src, tab.p + N, tab.p, tab.p + 2 * N, dzp,
-         beta1_delay, z, z_src, nZP, p->pumpF[q].p, p->pumpT[q].p, p->tw, p->stream);
+  if (p->L == 0) {
+    if (int e = nlm_any_pump_import(p, q, src->gridF.p + (size_t)mode * src->nZ * N, src->nZ, dz_src, tab.p + N,
+                                    tab.p, tab.p + 2 * N, dzp, beta1_delay, z, z_src))
+      return e;
+  } else {
+    const import_fn launch = pick_import(p->L);
+    if (!launch) return set_err(NLM_INVALID_ARGUMENT, "unsupported size for the device pump import");
+    launch(src->gridF.p + (size_t)mode * src->nZ * N, src->nZ, dz_src, tab.p + N, tab.p, tab.p + 2 * N, dzp,
+           beta1_delay, z, z_src, nZP, p->pumpF[q].p, p->pumpT[q].p, p->tw, p->stream);
+  }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(p->stream));  // before the temporary tables are freed
   p->pumpLinear = false;
@@ -705,10 +722,11 @@ void input_channels(int nS, int nCh, int inputMode, int* inPos) {
 }
 
 bool uses_step_table(const nlm_plan* p) {
-  return p->stepTable && (p->model == NLM_CHI2PDC || p->model == NLM_CHI3);
+  return p->L != 0 && p->stepTable && (p->model == NLM_CHI2PDC || p->model == NLM_CHI3);
 }
 
 int launch_prop_kernel(nlm_plan* p, const KParams& kp0) {
+  if (p->L == 0) return nlm_any_prop(p, kp0);
   cplx* tab = nullptr;
   if (uses_step_table(p)) {
     if (int e = p->stepTab.ensure((size_t)p->nZ * p->N * 2)) return e;
@@ -768,6 +786,7 @@ int tune_variant(nlm_plan* p, const KParams& kp, int grid, int* best) {
 }  // namespace
 
 int launch_prop_kernel_tab(nlm_plan* p, const KParams& kp0, cplx* tab, bool may_tune) {
+  if (p->L == 0) return nlm_any_prop(p, kp0, false);
   if (p->L > 13)
     return set_err(NLM_INVALID_ARGUMENT, "nFreqs above 8192 is available for Chi3GNLSE batchSignalSimulation only");
   KParams kp = kp0;
@@ -1040,13 +1059,15 @@ int nlm_fft_device(const cplx* in, cplx* out, int n, int count, bool inverse, hi
 }
 
 extern "C" int nlm_fft(int n, int count, const double* in, double* out, int inverse) {
-  int L;
-  if (int e = check_n(n, &L)) return e;
+  int L = 0;
+  const bool any = nlm_any_size(n);
+  if (!any)
+    if (int e = check_n(n, &L)) return e;
   if (count < 0 || (count > 0 && (!in || !out))) return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
   if (count == 0) return NLM_OK;
   const cplx* tw = nullptr;
   if (int e = get_twiddles(n, &tw)) return e;
-  DevBuf<cplx> a, b;
+  DevBuf<cplx> a, b, ws;
   const size_t cnt = (size_t)n * count;
   if (int e = a.ensure(cnt)) return e;
   if (int e = b.ensure(cnt)) {
@@ -1055,8 +1076,13 @@ extern "C" int nlm_fft(int n, int count, const double* in, double* out, int inve
   }
   hipError_t err = hipMemcpy(a.p, in, cnt * sizeof(cplx), hipMemcpyHostToDevice);
   if (err == hipSuccess) {
-    pick_fft(L)(a.p, b.p, count, tw, inverse != 0, 0);
-    err = hipGetLastError();
+    if (any) {
+      if (int e = nlm_any_fft(a.p, b.p, n, count, inverse != 0, 0, &ws)) return e;
+      err = hipDeviceSynchronize();  // before the workspace is freed
+    } else {
+      pick_fft(L)(a.p, b.p, count, tw, inverse != 0, 0);
+      err = hipGetLastError();
+    }
   }
   if (err == hipSuccess) err = hipMemcpy(out, b.p, cnt * sizeof(cplx), hipMemcpyDeviceToHost);
   a.release();

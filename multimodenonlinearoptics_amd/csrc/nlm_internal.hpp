@@ -73,6 +73,7 @@ struct nlm_plan_s {
   const void* bigIn = nullptr;
   const void* bigOut = nullptr;
   const void* bigStatePtr = nullptr;
+  DevBuf<cplx> anyScratch;  // workspaces of the any-length path (nlm_any.hip) when they exceed LDS
   DevBuf<cplx> stepTab;  // RK4 step propagators of one-mode undepleted media [nZ][N][2]
   int stepTable = 1;      // use them (nlm_plan_set_step_table)
   int ckptEvery = 0, nSeg = 0, gNIn = 0, gNOut = 0, gInTime = 0, gHasScale = 0;
@@ -101,6 +102,17 @@ void input_channels(int nS, int nCh, int inputMode, int* inPos);
 bool nlm_big_supported(int model, int L);
 int nlm_big_batch(nlm_plan_s* p, const cplx* in, int rows, int cols, int inTime, int inPos, int nOut, int outPos,
                   cplx* out);
+// Any even nFreqs outside the power-of-two kernels (nlm_any.hip): not a power of
+// two, 2 and 4, or above 8192.  Plans of these lengths have L = 0.
+bool nlm_any_size(int n);
+// may_alloc = false: fail instead of allocating the device workspace that
+// lengths above the LDS workspace need
+int nlm_any_prop(nlm_plan_s* p, const KParams& kp, bool may_alloc = true);
+int nlm_any_pump_linear(nlm_plan_s* p, const double* strength_dev, double dzp);
+int nlm_any_pump_chi3(nlm_plan_s* p, double dzp);
+int nlm_any_pump_import(nlm_plan_s* p, int q, const cplx* sigF, int nzs, double dzs, const double* Ds,
+                        const double* Dp, const double* omega, double dzp, double bd, double z, double zs);
+int nlm_any_fft(const cplx* in, cplx* out, int n, int count, bool inverse, hipStream_t s, DevBuf<cplx>* scratch);
 void tic(nlm_plan_s* p, int slot);
 void collect_timing(nlm_plan_s* p, int a0, int a1, int slot);
 void launch_greens_assemble(const cplx* G, cplx* C, cplx* S, int N, int nIn, int nOut, int nb, const cplx* scale,

@@ -1,0 +1,150 @@
+"""nFreqs outside the power-of-two kernels: lengths that are not powers of two (the
+reference accepts any even nFreqs, src/_NonlinearMedium.cpp:108-109, transformed
+by kissfft's mixed-radix FFT, src/CustomEigenFFT.h:17-40), 2 and 4, and lengths
+above 8192.  The engine runs them through its any-length path (nlm_any.hip);
+every result is compared with the oracle (whose own mixed-radix transform is
+pinned to numpy in tests/test_oracle_pinned.py) at rel. L2 <= 1e-10, and the
+transform itself with numpy."""
+import numpy as np
+import pytest
+
+from cases import DEPLETED, UNDEPLETED, make, random_inputs, small_case
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from multimodenonlinearoptics_amd import capi as c
+    return c
+
+
+@pytest.mark.parametrize("n", [2, 4, 6, 10, 12, 24, 30, 100, 250, 998, 1000, 3000, 10000, 16384, 20000])
+def test_fft_any_length(capi, n):
+    print("fft n=%d" % n, flush=True)
+    r = np.random.default_rng(n)
+    x = r.standard_normal((3, n)) + 1j * r.standard_normal((3, n))
+    y = capi.fft(x)
+    assert rel_l2(y, np.fft.fft(x, axis=-1)) < 1e-13
+    assert rel_l2(capi.fft(x, inverse=True), np.fft.ifft(x, axis=-1)) < 1e-13
+
+
+NO_GNLSE = [m for m in UNDEPLETED + DEPLETED if m != "Chi3GNLSE"]
+
+
+@pytest.mark.parametrize("name", NO_GNLSE)
+@pytest.mark.parametrize("in_time", [False, True])
+def test_batch_any_length(nlm, oracle, name, in_time):
+    N, nZ, rows = 30, 24, 3
+    print("batch %s N=%d in_time=%s" % (name, N, in_time), flush=True)
+    _, kw = small_case(name, nlm, N=N, nZ=nZ, seed=sum(map(ord, name)))
+    _, kwo = small_case(name, oracle, N=N, nZ=nZ, seed=sum(map(ord, name)))
+    dev, ref = make(nlm, name, kw), make(oracle, name, kwo)
+    X = random_inputs(name, N, rows, seed=5)
+    extra = {} if name in DEPLETED else {"runPump": True}
+    got = dev.batchSignalSimulation(X, inTimeDomain=in_time, **extra)
+    exp = ref.batchSignalSimulation(X, inTimeDomain=in_time, **extra)
+    assert got.shape == exp.shape
+    assert rel_l2(got, exp) < TOL
+
+
+@pytest.mark.parametrize("name", ["Chi3", "Chi2PDC", "Chi2SFG", "Chi2SFGOPA"])
+@pytest.mark.parametrize("N", [2, 4, 6, 100])
+def test_greens_any_length(nlm, oracle, name, N):
+    nZ = 30
+    print("greens %s N=%d" % (name, N), flush=True)
+    _, kw = small_case(name, nlm, N=N, nZ=nZ, seed=3)
+    _, kwo = small_case(name, oracle, N=N, nZ=nZ, seed=3)
+    dev, ref = make(nlm, name, kw), make(oracle, name, kwo)
+    for in_time in (False, True):
+        C, S = dev.computeGreensFunction(inTimeDomain=in_time)
+        Co, So = ref.computeGreensFunction(inTimeDomain=in_time)
+        assert rel_l2(C, Co) < TOL and rel_l2(S, So) < TOL
+
+
+def test_greens_chi2pdc_n1000(nlm, oracle):
+    N, nZ = 1000, 60
+    print("greens Chi2PDC N=%d nZ=%d" % (N, nZ), flush=True)
+    _, kw = small_case("Chi2PDC", nlm, N=N, nZ=nZ, seed=4)
+    _, kwo = small_case("Chi2PDC", oracle, N=N, nZ=nZ, seed=4)
+    C, S = make(nlm, "Chi2PDC", kw).computeGreensFunction()
+    Co, So = make(oracle, "Chi2PDC", kwo).computeGreensFunction()
+    assert rel_l2(C, Co) < TOL and rel_l2(S, So) < TOL
+
+
+@pytest.mark.parametrize("name,N", [("Chi3", 100), ("Chi2SFG", 60), ("Chi2SHG", 90)])
+def test_signal_grids_any_length(nlm, oracle, name, N):
+    nZ = 40
+    print("grid %s N=%d" % (name, N), flush=True)
+    _, kw = small_case(name, nlm, N=N, nZ=nZ, seed=9)
+    _, kwo = small_case(name, oracle, N=N, nZ=nZ, seed=9)
+    dev, ref = make(nlm, name, kw), make(oracle, name, kwo)
+    x = random_inputs(name, N, 1, seed=2)[0]
+    dev.runSignalSimulation(x, True)
+    ref.runSignalSimulation(x, True)
+    for q in range(1 if name == "Chi3" else 2):
+        assert rel_l2(np.asarray(dev.signalFreqs(q)), np.asarray(ref.signalFreqs(q))) < TOL
+        assert rel_l2(np.asarray(dev.signalTimes(q)), np.asarray(ref.signalTimes(q))) < TOL
+    if name in UNDEPLETED:
+        assert rel_l2(np.asarray(dev.pumpTime), np.asarray(ref.pumpTime)) < TOL
+        assert rel_l2(np.asarray(dev.pumpFreq), np.asarray(ref.pumpFreq)) < TOL
+
+
+def test_pump_import_any_length(nlm, oracle):
+    N = 60
+    print("pump import N=%d" % N, flush=True)
+    tau = 2 * 10.0 / N * np.fft.fftshift(np.arange(-N // 2, N // 2))
+    x = np.exp(-tau ** 2 / 2).astype(np.complex128)
+    res = []
+    for mod in (nlm, oracle):
+        a = mod.Chi2PDC(relativeLength=1.3, nlLength=1.0, beta2=0.3, beta2s=-0.4, beta1=0.2, beta1s=-0.5,
+                        diffBeta0=0.7, tMax=10, tPrecision=N, nZSteps=50)
+        b = mod.Chi2PDC(relativeLength=0.8, nlLength=1.0, beta2=-0.2, beta2s=0.5, beta1=-0.3, beta1s=0.4,
+                        diffBeta0=-1.2, tMax=10, tPrecision=N, nZSteps=37)
+        a.runPumpSimulation()
+        a.runSignalSimulation(x, True)
+        b.setPump(a, 0, 0.25)
+        C, S = b.computeGreensFunction(runPump=False)
+        res.append((np.array(b.pumpFreq), np.array(b.pumpTime), C, S))
+    for got, exp in zip(res[0], res[1]):
+        assert rel_l2(got, exp) < TOL
+
+
+def test_cascade_any_length(nlm, oracle):
+    N = 48 * 3
+    print("cascade N=%d" % N, flush=True)
+    out = []
+    for mod in (nlm, oracle):
+        _, k1 = small_case("Chi2PDC", mod, N=N, nZ=25, seed=21)
+        _, k2 = small_case("Chi2PDC", mod, N=N, nZ=31, seed=22)
+        c = mod.Cascade([make(mod, "Chi2PDC", k1), make(mod, "Chi2PDC", k2)], [{0: 0}], False)
+        out.append(c.computeGreensFunction())
+    for got, exp in zip(out[0], out[1]):
+        assert rel_l2(got, exp) < TOL
+
+
+@pytest.mark.parametrize("name,N,nZ", [("Chi2PDC", 16384, 12), ("Chi3", 10000, 10), ("Chi2SHG", 12288, 10)])
+def test_lengths_beyond_lds(nlm, oracle, name, N, nZ):
+    """Workspaces beyond 64 KB of LDS live in the plan's device scratch."""
+    print("large %s N=%d nZ=%d" % (name, N, nZ), flush=True)
+    _, kw = small_case(name, nlm, N=N, nZ=nZ, seed=6)
+    _, kwo = small_case(name, oracle, N=N, nZ=nZ, seed=6)
+    dev, ref = make(nlm, name, kw), make(oracle, name, kwo)
+    X = random_inputs(name, N, 2, seed=8)
+    extra = {} if name in DEPLETED else {"runPump": True}
+    got = dev.batchSignalSimulation(X, inTimeDomain=True, **extra)
+    exp = ref.batchSignalSimulation(X, inTimeDomain=True, **extra)
+    assert rel_l2(got, exp) < TOL
+
+
+def test_paths_that_need_powers_of_two(nlm):
+    _, kw = small_case("Chi3GNLSE", nlm, N=30, nZ=10, seed=1)
+    g = make(nlm, "Chi3GNLSE", kw)
+    with pytest.raises(ValueError, match="power of two"):
+        g.batchSignalSimulation(np.ones((1, 30), dtype=complex))
+    _, kw = small_case("Chi2PDC", nlm, N=30, nZ=10, seed=1)
+    m = make(nlm, "Chi2PDC", kw)
+    with pytest.raises(ValueError, match="power of two"):
+        m.greensForward()
