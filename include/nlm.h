@@ -25,8 +25,8 @@
  *  - transforms follow src/CustomEigenFFT.h:54-137: forward unscaled,
  *    inverse scaled by 1/n.  Any even n is supported: n = 2^k, 8 <= n <= 8192
  *    runs the in-register kernels, every other even n (and Chi3GNLSE batches
- *    of 16384 and 32768) the mixed-radix and staged paths; Chi3GNLSE and the
- *    adjoints need powers of two.
+ *    of 16384 and 32768) the mixed-radix and staged paths; the adjoints and
+ *    heterogeneous batches need powers of two.
  *  - a plan is not safe for concurrent calls; distinct plans may be used from
  *    distinct threads.
  */

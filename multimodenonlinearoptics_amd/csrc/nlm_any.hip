@@ -16,7 +16,8 @@
 // forward unscaled, inverse scaled by 1/N.
 //
 // Replaced here for these lengths: signalSimulationTemplate
-// (src/_NonlinearMedium.hpp:160-239) with every pointwise solver DiffEq, the
+// (src/_NonlinearMedium.hpp:160-239) with every solver DiffEq (Chi3GNLSE's
+// transforms inside each RK stage included, solvers/Chi3GNLSE.cpp:51-92), the
 // linear pump (src/_NonlinearMedium.cpp:225-250), Chi3::runPumpSimulation
 // (solvers/Chi3.cpp:25-54), setPump(other) (src/_NonlinearMedium.cpp:506-534)
 // and the transforms of nlm_fft.
@@ -100,17 +101,15 @@ __device__ cplx* any_row(const KParams& kp, cplx* base, int field, int nS, int q
   return base + (((size_t)field * nS + q) * kp.nZ + i) * N;
 }
 
-// One propagation of field `field` (signalSimulationTemplate) over the mode
-// buffers cur[q] / oth[q]; pointwise phases use the lane-to-point mapping
-// k = threadIdx.x + blockDim.x * j throughout, so they need no barrier between
-// them, and every transform synchronizes on entry and exit.
-template <class M>
-__device__ void any_field(const KParams& kp, const AnyFft& f, int field, cplx* (&cur)[M::NS], cplx* (&oth)[M::NS]) {
-  constexpr int NS = M::NS, NP = M::NP, NC = M::NC;
-  constexpr int NPP = NP > 0 ? NP : 1;
+// Inputs and the half dispersion step in (src/_NonlinearMedium.hpp:164-198,
+// src/_NonlinearMedium.cpp:329-340): leaves the time-domain field in cur[q]
+// (frequency domain when nZ == 1) and writes the grids' row 0.  Pointwise phases
+// use the lane-to-point mapping k = threadIdx.x + blockDim.x * j throughout, so
+// they need no barrier between them; every transform synchronizes on entry and exit.
+template <int NS>
+__device__ void any_prologue(const KParams& kp, const AnyFft& f, int field, cplx* (&cur)[NS], cplx* (&oth)[NS]) {
   const int N = f.N, tid = threadIdx.x, nt = blockDim.x;
   __syncthreads();  // the previous field's readers are done
-  // inputs (src/_NonlinearMedium.hpp:164-198, src/_NonlinearMedium.cpp:329-340)
   for (int q = 0; q < NS; q++) {
     if (kp.kind == KIND_GREENS) {
       // field 2i + w: impulse e_(i mod N) (w = 0) or i e_(i mod N) (w = 1) into mode inputs[i / N]
@@ -124,7 +123,6 @@ __device__ void any_field(const KParams& kp, const AnyFft& f, int field, cplx* (
       for (int k = tid; k < N; k += nt) cur[q][k] = mk(0.0, 0.0);
     }
   }
-  // half dispersion step in; the grid's row 0
   for (int q = 0; q < NS; q++) {
     if (kp.inTime) mode_fft<false>(f, cur[q], oth[q]);
     mul_any(cur[q], kp.halfIn + (size_t)q * N, N);
@@ -135,6 +133,46 @@ __device__ void any_field(const KParams& kp, const AnyFft& f, int field, cplx* (
       mode_fft<true>(f, cur[q], oth[q]);
       if (kp.kind == KIND_GRID) store_any(any_row(kp, kp.gridT, field, NS, q, 0, N), cur[q], N);
     }
+}
+
+// Dispersion step i of mode q (src/_NonlinearMedium.hpp:212-215) with the grids' row i.
+__device__ void any_dispersion(const KParams& kp, const AnyFft& f, int field, int NS, int q, int i, cplx*& cur,
+                               cplx*& oth) {
+  const int N = f.N;
+  mode_fft<false>(f, cur, oth);
+  mul_any(cur, kp.dstep + (size_t)q * N, N);
+  if (i < kp.nZ - 1) {
+    if (kp.kind == KIND_GRID) store_any(any_row(kp, kp.gridF, field, NS, q, i, N), cur, N);
+    mode_fft<true>(f, cur, oth);
+    if (kp.kind == KIND_GRID) store_any(any_row(kp, kp.gridT, field, NS, q, i, N), cur, N);
+  }
+}
+
+// Half step out and outputs (src/_NonlinearMedium.hpp:235-238, src/_NonlinearMedium.cpp:333-345, 428-431).
+template <int NS>
+__device__ void any_epilogue(const KParams& kp, const AnyFft& f, int field, cplx* (&cur)[NS], cplx* (&oth)[NS]) {
+  const int N = f.N, last = kp.nZ - 1;
+  for (int q = 0; q < NS; q++) {
+    mul_any(cur[q], kp.halfOut + (size_t)q * N, N);
+    if (kp.kind == KIND_GRID) {
+      store_any(any_row(kp, kp.gridF, field, NS, q, last, N), cur[q], N);
+      mode_fft<true>(f, cur[q], oth[q]);
+      store_any(any_row(kp, kp.gridT, field, NS, q, last, N), cur[q], N);
+    } else {
+      if (kp.outTime) mode_fft<true>(f, cur[q], oth[q]);
+      if (kp.outPos[q] >= 0)
+        store_any(kp.out + (size_t)field * kp.nOut * N + (size_t)kp.outPos[q] * N, cur[q], N);
+    }
+  }
+}
+
+// One propagation of field `field` (signalSimulationTemplate) for a pointwise DiffEq.
+template <class M>
+__device__ void any_field(const KParams& kp, const AnyFft& f, int field, cplx* (&cur)[M::NS], cplx* (&oth)[M::NS]) {
+  constexpr int NS = M::NS, NP = M::NP, NC = M::NC;
+  constexpr int NPP = NP > 0 ? NP : 1;
+  const int N = f.N, tid = threadIdx.x, nt = blockDim.x;
+  any_prologue<NS>(kp, f, field, cur, oth);
   const cplx* pumps[2] = {kp.pump0, kp.pump1};
   for (int i = 1; i < kp.nZ; i++) {
     // RK4 nonlinear step (src/_NonlinearMedium.hpp:207-210), pointwise
@@ -156,31 +194,63 @@ __device__ void any_field(const KParams& kp, const AnyFft& f, int field, cplx* (
       M::template f<3>(c + 2 * NC, kp.cconst, y, P2, k4);
       for (int q = 0; q < NS; q++) cur[q][k] = x[q] + (((k1[q] + 2.0 * k2[q]) + 2.0 * k3[q]) + k4[q]) * (1. / 6.);
     }
-    // dispersion step (src/_NonlinearMedium.hpp:212-215)
-    for (int q = 0; q < NS; q++) {
-      mode_fft<false>(f, cur[q], oth[q]);
-      mul_any(cur[q], kp.dstep + (size_t)q * N, N);
-      if (i < kp.nZ - 1) {
-        if (kp.kind == KIND_GRID) store_any(any_row(kp, kp.gridF, field, NS, q, i, N), cur[q], N);
-        mode_fft<true>(f, cur[q], oth[q]);
-        if (kp.kind == KIND_GRID) store_any(any_row(kp, kp.gridT, field, NS, q, i, N), cur[q], N);
-      }
-    }
+    for (int q = 0; q < NS; q++) any_dispersion(kp, f, field, NS, q, i, cur[q], oth[q]);
   }
-  // half step out and outputs (src/_NonlinearMedium.hpp:235-238, src/_NonlinearMedium.cpp:333-345, 428-431)
-  const int last = kp.nZ - 1;
-  for (int q = 0; q < NS; q++) {
-    mul_any(cur[q], kp.halfOut + (size_t)q * N, N);
-    if (kp.kind == KIND_GRID) {
-      store_any(any_row(kp, kp.gridF, field, NS, q, last, N), cur[q], N);
-      mode_fft<true>(f, cur[q], oth[q]);
-      store_any(any_row(kp, kp.gridT, field, NS, q, last, N), cur[q], N);
-    } else {
-      if (kp.outTime) mode_fft<true>(f, cur[q], oth[q]);
-      if (kp.outPos[q] >= 0)
-        store_any(kp.out + (size_t)field * kp.nOut * N + (size_t)kp.outPos[q] * N, cur[q], N);
+  any_epilogue<NS>(kp, f, field, cur, oth);
+}
+
+// Chi3GNLSE stage increment (solvers/Chi3GNLSE.cpp:51-92):
+// k = IFFT( FFT( IFFT(FFT(r |y|^2) * R) * y ) * (nl0 + nl1*omega) ), into K (KO its
+// ping-pong partner); y is not modified.
+__device__ void any_gnlse_stage(const KParams& kp, const AnyFft& f, const cplx* y, double r, cplx*& K, cplx*& KO) {
+  const int N = f.N;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) K[k] = mk(r * abs2(y[k]), 0.0);
+  mode_fft<false>(f, K, KO);
+  mul_any(K, kp.raman, N);
+  mode_fft<true>(f, K, KO);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) K[k] = K[k] * y[k];
+  mode_fft<false>(f, K, KO);
+  mul_any(K, kp.filt, N);
+  mode_fft<true>(f, K, KO);
+}
+
+// One Chi3GNLSE propagation; workspace a, its transform partner, y, acc, k, k's partner.
+__device__ void any_gnlse_field(const KParams& kp, const AnyFft& f, int field, cplx* ws) {
+  const int N = f.N, tid = threadIdx.x, nt = blockDim.x;
+  cplx* cur[1] = {ws};
+  cplx* oth[1] = {ws + N};
+  cplx* Y = ws + 2 * (size_t)N;
+  cplx* acc = ws + 3 * (size_t)N;
+  cplx* K = ws + 4 * (size_t)N;
+  cplx* KO = ws + 5 * (size_t)N;
+  any_prologue<1>(kp, f, field, cur, oth);
+  for (int i = 1; i < kp.nZ; i++) {
+    const cplx* c = kp.coef + (size_t)i * 3;
+    const double r0 = c[0].re, r1 = c[1].re, r2 = c[2].re;
+    cplx* A = cur[0];
+    // acc accumulates ((k1 + 2 k2) + 2 k3) + k4 in the reference's order
+    any_gnlse_stage(kp, f, A, r0, K, KO);
+    for (int k = tid; k < N; k += nt) {
+      acc[k] = K[k];
+      Y[k] = A[k] + 0.5 * K[k];
     }
+    any_gnlse_stage(kp, f, Y, r1, K, KO);
+    __syncthreads();  // every lane has read Y before it is rewritten
+    for (int k = tid; k < N; k += nt) {
+      acc[k] = acc[k] + 2.0 * K[k];
+      Y[k] = A[k] + 0.5 * K[k];
+    }
+    any_gnlse_stage(kp, f, Y, r1, K, KO);
+    __syncthreads();
+    for (int k = tid; k < N; k += nt) {
+      acc[k] = acc[k] + 2.0 * K[k];
+      Y[k] = A[k] + K[k];
+    }
+    any_gnlse_stage(kp, f, Y, r2, K, KO);
+    for (int k = tid; k < N; k += nt) A[k] = A[k] + (acc[k] + K[k]) * (1. / 6.);
+    any_dispersion(kp, f, field, 1, 0, i, cur[0], oth[0]);
   }
+  any_epilogue<1>(kp, f, field, cur, oth);
 }
 
 // Workspace of one workgroup: 2 NS buffers of N points, in LDS (dynamic shared
@@ -202,6 +272,11 @@ __global__ void __launch_bounds__(ANY_THREADS) k_any_prop(KParams kp, AnyFft f, 
     }
     any_field<M>(kp, f, field, cur, oth);
   }
+}
+
+__global__ void __launch_bounds__(ANY_THREADS) k_any_gnlse(KParams kp, AnyFft f, cplx* gws) {
+  cplx* ws = any_workspace(gws, 3, f.N);
+  for (int field = blockIdx.x; field < kp.nFields; field += gridDim.x) any_gnlse_field(kp, f, field, ws);
 }
 
 // Rows of `count` transforms (nlm_fft; forward unscaled, inverse scaled by 1/N).
@@ -390,9 +465,13 @@ void launch_any_prop(const KParams& kp, const AnyFft& f, const AnyLaunch& al, hi
   hipLaunchKernelGGL(k_any_prop<M>, dim3(al.grid), dim3(ANY_THREADS), al.lds, s, kp, f, al.gws);
 }
 
+void launch_any_gnlse(const KParams& kp, const AnyFft& f, const AnyLaunch& al, hipStream_t s) {
+  hipLaunchKernelGGL(k_any_gnlse, dim3(al.grid), dim3(ANY_THREADS), al.lds, s, kp, f, al.gws);
+}
+
 template <int MODEL>
 any_prop_fn any_prop_of() {
-  if constexpr (MODEL == NLM_CHI3GNLSE) return nullptr;
+  if constexpr (MODEL == NLM_CHI3GNLSE) return &launch_any_gnlse;
   else return &launch_any_prop<typename ModelOf<MODEL>::type>;
 }
 
@@ -412,6 +491,7 @@ any_prop_fn pick_any_prop(int model) {
     case NLM_CHI2DSFG: return any_prop_of<NLM_CHI2DSFG>();
     case NLM_CHI2SHGXPM: return any_prop_of<NLM_CHI2SHGXPM>();
     case NLM_CHI2ASHG: return any_prop_of<NLM_CHI2ASHG>();
+    case NLM_CHI3GNLSE: return any_prop_of<NLM_CHI3GNLSE>();
   }
   return nullptr;
 }
@@ -422,14 +502,15 @@ bool nlm_any_size(int n) { return n >= 2 && n % 2 == 0 && (n < 8 || n > 8192 || 
 
 int nlm_any_prop(nlm_plan_s* p, const KParams& kp, bool may_alloc) {
   const any_prop_fn fn = pick_any_prop(p->model);
-  if (!fn)
-    return set_err(NLM_INVALID_ARGUMENT,
-                   "Chi3GNLSE needs nFreqs a power of two (8..8192; batches also 16384 and 32768) on the MI355X path");
+  if (!fn) return set_err(NLM_INVALID_ARGUMENT, "unsupported model");
   if (kp.nFields <= 0) return NLM_OK;
   AnyFft f;
   if (int e = any_plan(p->N, &f)) return e;
   AnyLaunch al;
-  if (int e = any_launch(p, nullptr, p->nS, p->N, kp.nFields, &al, may_alloc)) return e;
+  // a Chi3GNLSE field needs six buffers (the state, y, the increment sum, the
+  // increment and two transform partners), a pointwise model two per mode
+  const int bufPairs = p->model == NLM_CHI3GNLSE ? 3 : p->nS;
+  if (int e = any_launch(p, nullptr, bufPairs, p->N, kp.nFields, &al, may_alloc)) return e;
   fn(kp, f, al, p->stream);
   HIP_TRY(hipGetLastError());
   return NLM_OK;

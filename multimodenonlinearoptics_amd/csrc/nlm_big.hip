@@ -566,7 +566,7 @@ static int gnlse_diff_check(nlm_plan* p) {
   if (!p) return set_err(NLM_INVALID_ARGUMENT, "null argument");
   if (p->model != NLM_CHI3GNLSE)
     return set_err(NLM_INVALID_ARGUMENT, "the differentiable batch is implemented for Chi3GNLSE");
-  if (!pick_stager(p->L).mix) return set_err(NLM_INVALID_ARGUMENT, "the differentiable batch needs nFreqs 32..32768");
+  if (!pick_stager(p->L).mix) return set_err(NLM_INVALID_ARGUMENT, "the differentiable batch needs nFreqs a power of two, 32..32768");
   if (p->nZ < 2) return set_err(NLM_INVALID_ARGUMENT, "the differentiable batch needs at least two z steps");
   return NLM_OK;
 }

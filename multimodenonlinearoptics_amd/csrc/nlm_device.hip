@@ -435,7 +435,7 @@ extern "C" int nlm_plan_create(const nlm_plan_desc* d, nlm_plan** out) {
   int L;
   if (d->model == NLM_CHI3GNLSE && (d->n_freqs == 16384 || d->n_freqs == 32768)) {
     L = log2_exact(d->n_freqs);  // staged transform (batches only)
-  } else if (d->model != NLM_CHI3GNLSE && nlm_any_size(d->n_freqs)) {
+  } else if (nlm_any_size(d->n_freqs)) {
     L = 0;  // any-length path (nlm_any.hip)
   } else if (int e = check_n(d->n_freqs, &L)) {
     return e;

@@ -1,7 +1,7 @@
 """nFreqs outside the power-of-two kernels: lengths that are not powers of two (the
 reference accepts any even nFreqs, src/_NonlinearMedium.cpp:108-109, transformed
 by kissfft's mixed-radix FFT, src/CustomEigenFFT.h:17-40), 2 and 4, and lengths
-above 8192.  The engine runs them through its any-length path (nlm_any.hip);
+above 8192, for every solver.  The engine runs them through its any-length path (nlm_any.hip);
 every result is compared with the oracle (whose own mixed-radix transform is
 pinned to numpy in tests/test_oracle_pinned.py) at rel. L2 <= 1e-10, and the
 transform itself with numpy."""
@@ -31,10 +31,7 @@ def test_fft_any_length(capi, n):
     assert rel_l2(capi.fft(x, inverse=True), np.fft.ifft(x, axis=-1)) < 1e-13
 
 
-NO_GNLSE = [m for m in UNDEPLETED + DEPLETED if m != "Chi3GNLSE"]
-
-
-@pytest.mark.parametrize("name", NO_GNLSE)
+@pytest.mark.parametrize("name", UNDEPLETED + DEPLETED)
 @pytest.mark.parametrize("in_time", [False, True])
 def test_batch_any_length(nlm, oracle, name, in_time):
     N, nZ, rows = 30, 24, 3
@@ -139,11 +136,32 @@ def test_lengths_beyond_lds(nlm, oracle, name, N, nZ):
     assert rel_l2(got, exp) < TOL
 
 
+@pytest.mark.parametrize("N,nZ", [(90, 30), (1000, 12), (12000, 6)])
+def test_gnlse_any_length(nlm, oracle, N, nZ):
+    """Chi3GNLSE: transforms inside every RK stage (solvers/Chi3GNLSE.cpp:51-92);
+    N = 12000 runs from the device workspace."""
+    print("gnlse N=%d nZ=%d" % (N, nZ), flush=True)
+    _, kw = small_case("Chi3GNLSE", nlm, N=N, nZ=nZ, seed=12)
+    _, kwo = small_case("Chi3GNLSE", oracle, N=N, nZ=nZ, seed=12)
+    # a time step of at least 0.02: finer grids reach frequencies where the
+    # self-steepening term makes the RK4 step unstable (the oracle too)
+    kw["tMax"] = kwo["tMax"] = max(kw["tMax"], 0.01 * N)
+    dev, ref = make(nlm, "Chi3GNLSE", kw), make(oracle, "Chi3GNLSE", kwo)
+    X = random_inputs("Chi3GNLSE", N, 2, seed=13)
+    exp = ref.batchSignalSimulation(X, inTimeDomain=True)
+    assert np.isfinite(exp).all()
+    assert rel_l2(dev.batchSignalSimulation(X, inTimeDomain=True), exp) < TOL
+    dev.runSignalSimulation(X[0], True)
+    ref.runSignalSimulation(X[0], True)
+    assert rel_l2(np.asarray(dev.signalFreq), np.asarray(ref.signalFreq)) < TOL
+    assert rel_l2(np.asarray(dev.signalTime), np.asarray(ref.signalTime)) < TOL
+
+
 def test_paths_that_need_powers_of_two(nlm):
     _, kw = small_case("Chi3GNLSE", nlm, N=30, nZ=10, seed=1)
     g = make(nlm, "Chi3GNLSE", kw)
     with pytest.raises(ValueError, match="power of two"):
-        g.batchSignalSimulation(np.ones((1, 30), dtype=complex))
+        g.batchForward(np.ones((1, 30), dtype=complex))
     _, kw = small_case("Chi2PDC", nlm, N=30, nZ=10, seed=1)
     m = make(nlm, "Chi2PDC", kw)
     with pytest.raises(ValueError, match="power of two"):
