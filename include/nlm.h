@@ -181,10 +181,13 @@ int nlm_batch_dev(nlm_plan* plan, const double* inputs_dev, int rows, int cols, 
  * nFreqs, nZSteps, dispersion, pump), each the batchSignalSimulation of one
  * input row (src/_NonlinearMedium.cpp:378-444, src/_FullyNonlinearMedium.hpp:8-11),
  * run as one captured graph of per-(model, nFreqs) persistent kernels fed by
- * device work queues.  Supported: Chi2PDC (pump rows generated in the kernel
- * from the envelope, as in src/_NonlinearMedium.cpp:225-250), Chi2SHG, Chi2ASHG;
- * nFreqs 32..4096.  The plans must outlive the batch.  (Extension: no
- * counterpart in the reference, which runs instances one by one.) */
+ * device work queues.  Every solver; nFreqs a power of two, 32..4096.  With
+ * `envelope` set, the pump rows of a single linear pump are generated in the
+ * kernel from the envelope (src/_NonlinearMedium.cpp:225-250); with it NULL a
+ * pumped medium's device pump grids are read (they must have been computed:
+ * Chi3's nonlinear pump, Chi2SFGOPA's two pumps, or any grid set before).  The
+ * plans must outlive the batch.  (Extension: no counterpart in the reference,
+ * which runs instances one by one.) */
 typedef struct {
   nlm_plan* plan;
   const double* input;       /* cols complex values; copied to the device at creation */

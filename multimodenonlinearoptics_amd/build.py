@@ -72,7 +72,7 @@ def build(force=False, jobs=None, verbose=False):
         steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_big.hip"), "-o", o])
     o = os.path.join(OBJ, "nlm_multi.o")
     objs.append(o)
-    deps = _deps("nlm_multi.hip", "nlm_internal.hpp", *dev_hdrs)
+    deps = _deps("nlm_multi.hip", "nlm_multi.hpp", "nlm_internal.hpp", *dev_hdrs)
     if force or _stale(o, deps):
         steps.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "nlm_multi.hip"), "-o", o])
     o = os.path.join(OBJ, "nlm_compose.o")
@@ -104,6 +104,13 @@ def build(force=False, jobs=None, verbose=False):
         if force or _stale(o, deps):
             steps.append([HIPCC] + HIP_FLAGS + ["-DNLM_INST_MODEL=%d" % m, "-c",
                                                 os.path.join(CSRC, "nlm_prop_inst.hip"), "-o", o])
+    for m in PROP_MODELS:  # heterogeneous-batch kernels of each model
+        o = os.path.join(OBJ, "nlm_multi_%d.o" % m)
+        objs.append(o)
+        deps = _deps("nlm_multi_inst.hip", "nlm_multi.hpp", *dev_hdrs)
+        if force or _stale(o, deps):
+            steps.append([HIPCC] + HIP_FLAGS + ["-DNLM_INST_MODEL=%d" % m, "-c",
+                                                os.path.join(CSRC, "nlm_multi_inst.hip"), "-o", o])
     host_srcs = ["nlm_host.cpp", "nlm_pybind.cpp"]
     import pybind11
     host_flags = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-I" + INCLUDE, "-I" + CSRC,

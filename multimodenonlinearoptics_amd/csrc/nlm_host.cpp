@@ -1361,7 +1361,7 @@ MultiBatch::MultiBatch(const std::vector<Medium*>& media, const std::vector<std:
     it.input_mode = 0;
     it.n_out = md->nS_;
     for (int q = 0; q < md->nS_ && q < 4; q++) it.outputs[q] = q;
-    if (md->nP_ == 1 && md->model_ == NLM_CHI2PDC) {
+    if (md->nP_ == 1 && md->model_ != NLM_CHI3) {  // single linear pump: generated in the batch
       it.envelope = reinterpret_cast<const double*>(md->envelope_[0].data());
       it.disp_pump = md->dispPump_[0].data();
       it.dzp = md->dzp_;
@@ -1375,6 +1375,10 @@ MultiBatch::MultiBatch(const std::vector<Medium*>& media, const std::vector<std:
         }
         it.strength = strengths[i].data();
       }
+    } else if (md->nP_ > 0) {
+      // Chi3's nonlinear pump and Chi2SFGOPA's two pumps: the device pump grids
+      // of the medium, computed here as batchSignalSimulation(runPump=True) does
+      md->runPumpSimulation();
     }
     outLen_.push_back(md->nS_ * md->N_);
   }

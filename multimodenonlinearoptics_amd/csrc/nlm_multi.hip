@@ -6,10 +6,12 @@
 // src/_FullyNonlinearMedium.hpp:8-11).  Instances are grouped into classes by
 // (model, log2 N); every class runs as one persistent kernel whose workgroups
 // take instances from a device work queue (one atomic counter per class) in
-// order of decreasing cost nZ N log2 N, so long propagations start first and
-// short ones fill the tail.  For linear-pump media the pump rows are generated
+// order of decreasing nZ, so long propagations start first and short ones fill
+// the tail.  For media with a single linear pump the pump rows are generated
 // inside the kernel (InlinePump), so no (2nZ-1) x N pump grid is written or
-// read per instance.  All classes are captured into one hipGraph (one branch
+// read per instance; Chi3 (nonlinear pump) and Chi2SFGOPA (two pumps) read their
+// plan's device pump grids.  Every solver has a kernel (Chi3GNLSE with its
+// transforms inside the RK stages).  All classes are captured into one hipGraph (one branch
 // per class on its own stream, joined at the end); nlm_multi_launch replays it.
 //
 // The number of persistent workgroups of a class is chosen by a small
@@ -24,115 +26,41 @@
 #include <vector>
 
 #include "nlm_internal.hpp"
-#include "nlm_prop.hpp"
-
-struct MultiInst {
-  const cplx* dstep;
-  const cplx* halfIn;
-  const cplx* halfOut;
-  const cplx* coef;
-  const cplx* cconst;
-  const cplx* in;
-  cplx* out;
-  const cplx* env;
-  const double* Dp;
-  const double* strength;
-  double dzp;
-  int nZ, inTime, inCols, nOut;
-  int inPos[4], outPos[4];
-};
-
-template <int L>
-struct MultiLaunch {
-  static constexpr int THREADS = FftShape<L>::T;  // one instance per workgroup
-};
-
-template <class M, int L, bool INLINE>
-__global__ void __launch_bounds__(MultiLaunch<L>::THREADS)
-    k_prop_multi(const MultiInst* __restrict__ inst, const int* __restrict__ order, int n, int* counter,
-                 const cplx* tw) {
-  using F = FftShape<L>;
-  __shared__ cplx lds[F::lds_elems];
-  __shared__ cplx twl[PropTw<L>::LDS_ELEMS];
-  __shared__ int item;
-  const int t = threadIdx.x;
-  fill_twiddle_lds<L>(twl, tw);
-  for (;;) {
-    __syncthreads();  // the previous instance's readers of `item` are done
-    if (t == 0) item = atomicAdd(counter, 1);
-    __syncthreads();
-    const int w = __builtin_amdgcn_readfirstlane(item);
-    if (w >= n) break;  // uniform: every workgroup leaves once the queue is empty
-    const MultiInst& I = inst[order[w]];
-    KParams kp{};
-    kp.nZ = I.nZ;
-    kp.kind = KIND_BATCH;
-    kp.nFields = 1;
-    kp.inTime = I.inTime;
-    kp.outTime = I.inTime;
-    kp.inCols = I.inCols;
-    kp.nOut = I.nOut;
-    for (int q = 0; q < 4; q++) {
-      kp.inPos[q] = I.inPos[q];
-      kp.outPos[q] = I.outPos[q];
-    }
-    kp.in = I.in;
-    kp.out = I.out;
-    kp.tw = tw;
-    kp.dstep = I.dstep;
-    kp.halfIn = I.halfIn;
-    kp.halfOut = I.halfOut;
-    kp.coef = I.coef;
-    kp.cconst = I.cconst;
-    kp.env = I.env;
-    kp.Dp = I.Dp;
-    kp.strength = I.strength;
-    kp.dzp = I.dzp;
-    prop_field<M, L, INLINE>(kp, 0, true, t, lds, twl);
-  }
-}
+#include "nlm_multi.hpp"
 
 __global__ void k_zero_ints(int* p, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = 0;
 }
 
-typedef void (*multi_fn)(const MultiInst*, const int*, int, int*, const cplx*, int grid, hipStream_t);
-
-template <class M, int L, bool INLINE>
-static void launch_multi(const MultiInst* inst, const int* order, int n, int* counter, const cplx* tw, int grid,
-                         hipStream_t s) {
-  hipLaunchKernelGGL((k_prop_multi<M, L, INLINE>), dim3(grid), dim3(MultiLaunch<L>::THREADS), 0, s, inst, order, n,
-                     counter, tw);
+// Kernel of a class: models sharing a right-hand side share kernels (Chi2AFC
+// runs Chi2SFG's, Chi2ASHG Chi2SHG's).
+static int class_model(int model) {
+  return model == NLM_CHI2AFC ? NLM_CHI2SFG : model == NLM_CHI2ASHG ? NLM_CHI2SHG : model;
 }
 
-#define NLM_MULTI_SWITCH(M, INL)                   \
-  switch (L) {                                     \
-    case 5: return &launch_multi<M, 5, INL>;       \
-    case 6: return &launch_multi<M, 6, INL>;       \
-    case 7: return &launch_multi<M, 7, INL>;       \
-    case 8: return &launch_multi<M, 8, INL>;       \
-    case 9: return &launch_multi<M, 9, INL>;       \
-    case 10: return &launch_multi<M, 10, INL>;     \
-    case 11: return &launch_multi<M, 11, INL>;     \
-    case 12: return &launch_multi<M, 12, INL>;     \
-  }                                                \
-  return nullptr;
-
-// Models with a heterogeneous-batch kernel: Chi2PDC (pump rows generated in
-// the kernel) and the second-harmonic media Chi2SHG / Chi2ASHG (no pump).
-static multi_fn pick_multi(int model, int L) {
-  if (model == NLM_CHI2PDC) {
-    NLM_MULTI_SWITCH(MChi2PDC, true)
-  }
-  if (model == NLM_CHI2SHG || model == NLM_CHI2ASHG) {
-    NLM_MULTI_SWITCH(MChi2SHG, false)
+static multi_fn pick_multi(int model, int L, bool inl) {
+  switch (class_model(model)) {
+    case NLM_CHI3: return nlm_pick_multi_0(L, inl);
+    case NLM_CHI2PDC: return nlm_pick_multi_1(L, inl);
+    case NLM_CHI2SFG: return nlm_pick_multi_2(L, inl);
+    case NLM_CHI2SFGPDC: return nlm_pick_multi_4(L, inl);
+    case NLM_CHI2SFGII: return nlm_pick_multi_5(L, inl);
+    case NLM_CHI2PDCII: return nlm_pick_multi_6(L, inl);
+    case NLM_CHI2SFGOPA: return nlm_pick_multi_7(L, inl);
+    case NLM_CHI2SFGXPM: return nlm_pick_multi_8(L, inl);
+    case NLM_CHI2SHG: return nlm_pick_multi_9(L, inl);
+    case NLM_CHI2SHGOPA: return nlm_pick_multi_10(L, inl);
+    case NLM_CHI2DSFG: return nlm_pick_multi_11(L, inl);
+    case NLM_CHI2SHGXPM: return nlm_pick_multi_12(L, inl);
+    case NLM_CHI3GNLSE: return nlm_pick_multi_14(L, inl);
   }
   return nullptr;
 }
 
 struct MultiClass {
   int model, L, count, grid;
+  bool inl;  // pump rows generated in the kernel
   multi_fn fn;
   size_t orderOff;  // offset of this class's instance order in the order array
   hipStream_t stream = nullptr;
@@ -156,7 +84,7 @@ struct nlm_multi_s {
 };
 
 static std::mutex g_tune_mu;
-static std::map<std::tuple<int, int, int>, int> g_tuned;  // (model, L, count) -> grid
+static std::map<std::tuple<int, int, int, int>, int> g_tuned;  // (model, L, generated pump, count) -> grid
 
 static int device_cus() {
   int dev = 0, cus = 256;
@@ -204,7 +132,7 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
 
   // sizes of the packed per-instance buffers
   size_t inTot = 0, outTot = 0, envTot = 0, realTot = 0;
-  std::map<std::pair<int, int>, std::vector<int>> byClass;
+  std::map<std::tuple<int, int, int>, std::vector<int>> byClass;  // (class model, L, generated pump)
   m->N.resize(n);
   m->nOut.resize(n);
   m->outOff.resize(n);
@@ -212,9 +140,14 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
     const nlm_multi_item& it = items[i];
     nlm_plan* p = it.plan;
     if (!p) return fail(set_err(NLM_INVALID_ARGUMENT, "null plan in item"));
-    if (!pick_multi(p->model, p->L))
+    const bool pumped = p->nP > 0;
+    const bool inl = pumped && it.envelope != nullptr;
+    if (inl && (p->nP != 1 || p->model == NLM_CHI3))
       return fail(set_err(NLM_INVALID_ARGUMENT,
-                          "heterogeneous batches support Chi2PDC, Chi2SHG and Chi2ASHG with nFreqs 32..4096"));
+                          "a pump generated in the batch needs a single linear pump; pass no envelope and run the "
+                          "medium's pump simulation instead"));
+    if (!pick_multi(p->model, p->L, inl))
+      return fail(set_err(NLM_INVALID_ARGUMENT, "heterogeneous batches take nFreqs a power of two, 32..4096"));
     if (!it.input || it.cols <= 0 || it.cols % p->N != 0 || it.cols / p->N > p->nS)
       return fail(set_err(NLM_INVALID_ARGUMENT, "Signals not of correct length!"));
     if (it.input_mode < 0 || it.input_mode >= p->nS)
@@ -223,19 +156,20 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
     for (int q = 0; q < it.n_out; q++)
       if (it.outputs[q] < 0 || it.outputs[q] >= p->nS)
         return fail(set_err(NLM_INVALID_ARGUMENT, "output mode out of range"));
-    const bool pumped = p->model == NLM_CHI2PDC;
-    if (pumped && (!it.envelope || !it.disp_pump))
-      return fail(set_err(NLM_INVALID_ARGUMENT, "a pumped medium needs its pump envelope and dispersion"));
+    if (inl && !it.disp_pump)
+      return fail(set_err(NLM_INVALID_ARGUMENT, "a generated pump needs the pump dispersion"));
+    if (pumped && !inl)
+      if (int e = check_pumps_ready(p)) return fail(e);
     m->N[i] = p->N;
     m->nOut[i] = it.n_out;
     m->outOff[i] = outTot;
     inTot += it.cols;
     outTot += (size_t)it.n_out * p->N;
-    if (pumped) {
+    if (inl) {
       envTot += p->N;
       realTot += p->N + (it.strength ? 2 * (size_t)p->nZ - 1 : 0);
     }
-    byClass[{p->model == NLM_CHI2ASHG ? NLM_CHI2SHG : p->model, p->L}].push_back(i);
+    byClass[std::make_tuple(class_model(p->model), p->L, inl ? 1 : 0)].push_back(i);
   }
   int e;
   if ((e = m->inBuf.ensure(inTot)) || (e = m->outBuf.ensure(outTot)) || (e = m->envBuf.ensure(envTot)) ||
@@ -269,7 +203,13 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
     for (int q = 0; q < 4; q++) d.inPos[q] = d.outPos[q] = -1;
     input_channels(p->nS, it.cols / p->N, it.input_mode, d.inPos);
     for (int q = 0; q < it.n_out; q++) d.outPos[it.outputs[q]] = q;
-    if (p->model == NLM_CHI2PDC) {
+    d.raman = p->raman.p;
+    d.filt = p->filt.p;
+    if (p->nP > 0 && !it.envelope) {
+      d.pump0 = p->pumpT[0].p;
+      d.pump1 = p->nP > 1 ? p->pumpT[1].p : nullptr;
+    }
+    if (p->nP > 0 && it.envelope) {
       std::memcpy(henv.data() + eo, it.envelope, sizeof(cplx) * p->N);
       d.env = m->envBuf.p + eo;
       eo += p->N;
@@ -289,9 +229,10 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
   hord.reserve(n);
   for (auto& kv : byClass) {
     MultiClass c;
-    c.model = kv.first.first;
-    c.L = kv.first.second;
-    c.fn = pick_multi(c.model, c.L);
+    c.model = std::get<0>(kv.first);
+    c.L = std::get<1>(kv.first);
+    c.inl = std::get<2>(kv.first) != 0;
+    c.fn = pick_multi(c.model, c.L, c.inl);
     std::vector<int> idx = kv.second;
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return hi[a].nZ > hi[b].nZ; });
     c.orderOff = hord.size();
@@ -316,7 +257,7 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
     if ((e = get_twiddles(1 << c.L, &tw))) return fail(e);
     const int cand[4] = {cus, 2 * cus, 4 * cus, 8 * cus};
     int best = std::min(c.count, 4 * cus);
-    const auto key = std::make_tuple(c.model, c.L, c.count);
+    const auto key = std::make_tuple(c.model, c.L, c.inl ? 1 : 0, c.count);
     bool cached = false;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -395,20 +395,12 @@ __device__ __forceinline__ void gnlse_stage(const cplx (&y)[8], double r, cplx (
   scale_inv<L>(k);
 }
 
+// One Chi3GNLSE propagation of field `field` on the T lanes that hold it.
 template <int L>
-__global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_gnlse(KParams kp) {
+__device__ __forceinline__ void gnlse_field(const KParams& kp, int field, bool active, int t, cplx* buf,
+                                            const cplx* twl) {
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T;
-  constexpr int FPB = Launch<L>::FPB;
-  __shared__ cplx lds[FPB * F::lds_elems];
-  __shared__ cplx twl[PropTw<L>::LDS_ELEMS];
-  fill_twiddle_lds<L>(twl, kp.tw);
-
-  const int lf = threadIdx.x / T, t = threadIdx.x % T;
-  const int field = blockIdx.x * FPB + lf;
-  const bool active = field < kp.nFields;
-  cplx* buf = lds + lf * F::lds_elems;
-
   typename PropTw<L>::type tw;
   init_twiddles<L>(tw, kp.tw, twl, t);
   cplx ds[8], ram[8], flt[8];
@@ -468,6 +460,17 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_gnlse(KParams kp) {
   epilogue<L, 1>(a, kp, buf, t, tw, field, active);
 }
 
+template <int L>
+__global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_gnlse(KParams kp) {
+  using F = FftShape<L>;
+  constexpr int T = F::T, FPB = Launch<L>::FPB;
+  __shared__ cplx lds[FPB * F::lds_elems];
+  __shared__ cplx twl[PropTw<L>::LDS_ELEMS];
+  fill_twiddle_lds<L>(twl, kp.tw);
+  const int lf = threadIdx.x / T, t = threadIdx.x % T;
+  const int field = blockIdx.x * FPB + lf;
+  gnlse_field<L>(kp, field, field < kp.nFields, t, lds + lf * F::lds_elems, twl);
+}
 
 typedef void (*prop_fn)(const KParams&, int grid, hipStream_t);
 

@@ -1,6 +1,6 @@
 """Heterogeneous batches (MultiBatch, nlm_multi_*): every instance against the
 oracle's batchSignalSimulation of the same medium and input (rel. L2 <= 1e-10),
-and against the engine's own per-medium path."""
+and against the engine's own per-medium path; every solver has a batch kernel."""
 import numpy as np
 import pytest
 
@@ -81,9 +81,9 @@ def test_large_pdc_class_matches_oracle(nlm, oracle):
 
 
 def test_mixed_batch_errors(nlm):
-    chi3 = nlm.Chi3(relativeLength=1, nlLength=1, beta2=-1, tMax=10, tPrecision=64, nZSteps=10)
-    with pytest.raises(ValueError, match="heterogeneous batches support"):
-        nlm.MultiBatch([chi3], [np.zeros(64, complex)])
+    big = nlm.Chi3(relativeLength=1, nlLength=1, beta2=-1, tMax=10, tPrecision=8192, nZSteps=10)
+    with pytest.raises(ValueError, match="power of two, 32..4096"):
+        nlm.MultiBatch([big], [np.zeros(8192, complex)])
     pdc = nlm.Chi2PDC(relativeLength=1, nlLength=1, beta2=0.1, beta2s=0.1, tMax=10, tPrecision=64, nZSteps=10)
     with pytest.raises(ValueError, match="Signals not of correct length"):
         nlm.MultiBatch([pdc], [np.zeros(60, complex)])
@@ -112,3 +112,37 @@ def test_config3_workload_matches_oracle(nlm, oracle):
         worst = max(worst, err)
         assert err < TOL, (i, insts[i][0], insts[i][1]["tPrecision"], insts[i][1]["nZSteps"], err)
     print("config 3: worst rel. L2 %.2e" % worst, flush=True)
+
+
+def test_every_solver_in_one_batch(nlm, oracle):
+    """All 15 solvers (two sizes each) in one heterogeneous batch: single linear
+    pumps generated in the kernel, Chi3's nonlinear pump and Chi2SFGOPA's two pumps
+    from the media's device grids, Chi3GNLSE with its in-stage transforms."""
+    from cases import ALL, make, random_inputs, small_case
+    media, ref, xs, names = [], [], [], []
+    for j, name in enumerate(ALL):
+        for N, nZ in ((32, 17 + j), (128, 40 - j)):
+            _, kw = small_case(name, nlm, N=N, nZ=nZ, seed=100 + j)
+            _, kwo = small_case(name, oracle, N=N, nZ=nZ, seed=100 + j)
+            media.append(make(nlm, name, kw))
+            ref.append(make(oracle, name, kwo))
+            xs.append(random_inputs(name, N, 1, seed=200 + j)[0])
+            names.append((name, N))
+    b = nlm.MultiBatch(media, xs, inTimeDomain=True, autotune=False)
+    b.run()
+    for i, (r, x) in enumerate(zip(ref, xs)):
+        print("instance", i, names[i], flush=True)
+        exp = r.batchSignalSimulation(x[None, :], inTimeDomain=True)[0]
+        got = b.output(i)
+        assert got.shape == exp.shape
+        assert rel_l2(got, exp) < TOL, (names[i], rel_l2(got, exp))
+    # classes: one per (kernel, size, generated pump); Chi2AFC shares Chi2SFG's, Chi2ASHG Chi2SHG's
+    assert sum(c[2] for c in b.classes()) == len(media)
+
+
+def test_batch_needs_power_of_two_lengths(nlm):
+    from cases import make, random_inputs, small_case
+    _, kw = small_case("Chi2PDC", nlm, N=48, nZ=10, seed=1)
+    m = make(nlm, "Chi2PDC", kw)
+    with pytest.raises(ValueError, match="power of two"):
+        nlm.MultiBatch([m], [random_inputs("Chi2PDC", 48, 1, seed=1)[0]], inTimeDomain=True)
