@@ -25,7 +25,10 @@ typedef void (*env_fn)(const cplx*, int, const double*, double, const double*, c
 template <int L>
 static void launch_sweep(const AParams& ap, int grid, hipStream_t s) {
   constexpr int FA = AdjShape<L>::FA;
-  hipLaunchKernelGGL((k_adj_sweep<L, FA>), dim3(grid), dim3(AdjShape<L>::THREADS), 0, s, ap);
+  if (ap.K == 1)
+    hipLaunchKernelGGL((k_adj_sweep<L, FA, false>), dim3(grid), dim3(AdjShape<L>::THREADS), 0, s, ap);
+  else
+    hipLaunchKernelGGL((k_adj_sweep<L, FA, true>), dim3(grid), dim3(AdjShape<L>::THREADS), 0, s, ap);
 }
 template <int L>
 static void launch_env(const cplx* gP, int nZP, const double* Dp, double dzp, const double* strength, const cplx* tw,

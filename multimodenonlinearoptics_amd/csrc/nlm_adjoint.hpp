@@ -16,7 +16,9 @@
 // and the table gradients g_alpha = sum_fields conj(a) h, g_beta = sum_fields a h.
 // The forward states are rebuilt on chip by inverting the step (dispersion is
 // unitary, the R-linear step is invertible: a = (conj(alpha) r - beta conj(r)) /
-// (|alpha|^2 - |beta|^2)), anchored at the checkpoints.
+// (|alpha|^2 - |beta|^2)), anchored at the checkpoints.  The table holds each
+// step as the real 2 x 2 matrix with columns R(1), R(i) (k_step_table), in which
+// g_a is its transpose and the inverse its adjugate over the determinant.
 // Each workgroup sums g_alpha, g_beta over its fields in LDS and writes one
 // partial per step; a second kernel (k_adj_reduce) sums the partials in a
 // fixed order and applies the chain rule from (alpha, beta) = (R(1) -+ i R(i))/2
@@ -40,7 +42,7 @@ struct AParams {
   const cplx* gout;    // [nFields][N] dL/d(output rows), domain = outTime
   cplx* state;         // [nFields][2][N] a_{i-1} and g_{a_i} between launches
   cplx* part;          // [nWG][chunk][2][N] workgroup sums of conj(a) h, a h
-  const cplx* tab;     // [nZ][N][2] RK4 step propagators (alpha, beta) / N
+  const cplx* tab;     // [nZ][N][2] RK4 step propagators as real matrices / N (k_step_table)
   const cplx* tw;
   const cplx* dstep;
   const cplx* halfOut;
@@ -143,26 +145,30 @@ NLM_DEV void prop_row(cplx (&a)[8], cplx (&b)[8], const cplx* __restrict__ row, 
   }
 }
 
-// x = R^-1(r) for the step r = alpha x + beta conj(x), given (alpha, beta) / N:
-// x = (conj(alpha) r - beta conj(r)) / (|alpha|^2 - |beta|^2).
+// x = R^-1(r) for the step r = R x, R the real matrix [[p.re, p.im], [q.re, q.im]] / N
+// acting on (Re x, Im x) (the table rows): x = adj(R) r / det(R), where det(R) N^2 =
+// |alpha|^2 - |beta|^2.
 template <int L>
-NLM_DEV void inverse_step(cplx (&x)[8], const cplx (&r)[8], const cplx (&a)[8], const cplx (&b)[8]) {
+NLM_DEV void inverse_step(cplx (&x)[8], const cplx (&r)[8], const cplx (&p)[8], const cplx (&q)[8]) {
   constexpr double N = FftShape<L>::N;
 #pragma unroll
   for (int m = 0; m < 8; m++) {
-    const double den = N * (abs2(a[m]) - abs2(b[m]));
+    const double den = N * (p[m].re * q[m].im - p[m].im * q[m].re);
     // hardware reciprocal refined by two Newton steps (relative error ~1 ulp;
-    // den is |alpha|^2 - |beta|^2 of a step close to the identity, far from 0
-    // and from the denormal range)
+    // den is the determinant of a step close to the identity, far from 0 and
+    // from the denormal range)
     double rd = __builtin_amdgcn_rcp(den);
     rd = rd * __builtin_fma(-den, rd, 2.0);
     rd = rd * __builtin_fma(-den, rd, 2.0);
-    x[m] = (conj(a[m]) * r[m] - b[m] * conj(r[m])) * rd;
+    x[m] = mk((q[m].im * r[m].re - p[m].im * r[m].im) * rd, (p[m].re * r[m].im - q[m].re * r[m].re) * rd);
   }
 }
 
 // Reverse sweep over steps iHi .. iLo+1 for FA fields per workgroup (T lanes each).
-template <int L, int FA>
+// RB = false: every state of the launch's steps is a kept checkpoint (K = 1), so no
+// state is rebuilt and the registers of the rebuild (the previous step's rows and
+// the state's transform) are not held.
+template <int L, int FA, bool RB>
 __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T, E = F::lds_elems;
@@ -237,12 +243,13 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
 #pragma unroll
     for (int m = 0; m < 8; m++) dsr[m] = conj(ap.dstep[t + T * m]) * (1.0 / N);
   }
-  // propagator rows (alpha, beta) / N of this step (al, be) and of the step
+  // propagator rows (matrix rows p, q) / N of this step (al, be) and of the step
   // before it (pa, pb, which also inverts it), loaded a step ahead so their
   // latency hides behind the transforms (at the point of use from N = 4096)
-  constexpr int PR = LEAN ? 1 : 8;
+  // (with RB false this step's rows are loaded at the top of the step instead)
+  constexpr int PR = LEAN || !RB ? 1 : 8;
   cplx al[8], be[8], pa[PR], pb[PR];
-  if constexpr (!LEAN) {
+  if constexpr (!LEAN && RB) {
     prop_row<L>(al, be, ap.tab + (size_t)ap.iHi * N * 2, t);
     if (ap.iHi >= 2) prop_row<L>(pa, pb, ap.tab + (size_t)(ap.iHi - 1) * N * 2, t);
   }
@@ -251,7 +258,8 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
     // adjoint of the dispersion step: h = g_r = IFFT(conj(exp(i dz D)) FFT(g));
     // with FUSE the state's dispersion step is undone in the same transforms:
     // xr = IFFT(conj(exp(i dz D)) FFT(a_{i-1})), from which a_{i-2} = R_{i-1}^-1(xr)
-    const bool rebuild = i >= 2 && (i - 2) % ap.K != 0;
+    const bool rebuild = RB && i >= 2 && (i - 2) % ap.K != 0;
+    if constexpr (!LEAN && !RB) prop_row<L>(al, be, ap.tab + (size_t)i * N * 2, t);
     const bool fused = FUSE && rebuild && !gFreq;
     cplx xr[8];
     if (fused) {
@@ -289,8 +297,9 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
     if constexpr (LEAN) prop_row<L>(al, be, ap.tab + (size_t)i * N * 2, t);
 #pragma unroll
     for (int m = 0; m < 8; m++) {
-      const cplx h = g[m], a = al[m] * (double)N, b = be[m] * (double)N;
-      g[m] = conj(a) * h + b * conj(h);
+      // the transpose of the step's real matrix: g = conj(alpha) h + beta conj(h)
+      const cplx h = g[m];
+      g[m] = mk((al[m].re * h.re + be[m].re * h.im) * (double)N, (al[m].im * h.re + be[m].im * h.im) * (double)N);
     }
 
     // state a_{i-2} for the next (earlier) step
@@ -307,14 +316,14 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
           for (int m = 0; m < 8; m++) xr[m] = (LEAN ? conj(ap.dstep[t + T * m]) * (1.0 / N) : dsr[LEAN ? 0 : m]) * xr[m];
           fft_field1<L, true>(xr, buf, t, tw);
         }
-        if constexpr (LEAN) {
+        if constexpr (LEAN || !RB) {
           prop_row<L>(al, be, ap.tab + (size_t)(i - 1) * N * 2, t);
           inverse_step<L>(x, xr, al, be);
         } else {
           inverse_step<L>(x, xr, pa, pb);
         }
       }
-      if constexpr (!LEAN) {
+      if constexpr (!LEAN && RB) {
 #pragma unroll
         for (int m = 0; m < 8; m++) {
           al[m] = pa[m];

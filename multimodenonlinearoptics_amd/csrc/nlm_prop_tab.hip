@@ -8,12 +8,14 @@
 // whole RK4 step at time sample k of step i is
 //     x -> alpha[i][k] x + beta[i][k] conj(x),
 // with alpha = (R(1) - i R(i)) / 2 and beta = (R(1) + i R(i)) / 2 for R the
-// RK4 step applied to the inputs 1 and i.  k_step_table evaluates R with the
-// model's own DiffEq functor once per (step, sample) for all fields; the
-// propagation kernel then spends 8 FP64 FMAs per point on the nonlinear step
-// instead of four DiffEq evaluations, for every one of the 2N Green's impulses
-// or batch rows.  The propagator also absorbs the inverse transform's 1/N
-// (an exact power of two), so the field between steps is kept unscaled.
+// RK4 step applied to the inputs 1 and i; as a real 2 x 2 matrix acting on
+// (Re x, Im x) its columns are R(1) and R(i).  k_step_table evaluates R with the
+// model's own DiffEq functor once per (step, sample) for all fields and stores
+// the matrix; the propagation kernel then spends 4 FP64 operations per point on
+// the nonlinear step instead of four DiffEq evaluations, for every one of the 2N
+// Green's impulses or batch rows.  The propagator also absorbs the inverse
+// transform's 1/N (an exact power of two), so the field between steps is kept
+// unscaled.
 #include "nlm_prop.hpp"
 
 template <class M>
@@ -30,7 +32,8 @@ __device__ __forceinline__ cplx rk4_step(const cplx* c, const cplx* cc, cplx x, 
   return x + (((k1[0] + 2.0 * k2[0]) + 2.0 * k3[0]) + k4[0]) * (1. / 6.);
 }
 
-// tab[(i*N + k)*2 + {0,1}] = {alpha, beta} * scale for steps i = 1..nZ-1.
+// tab[(i*N + k)*2 + {0,1}] = {(Re R(1), Re R(i)), (Im R(1), Im R(i))} * scale for
+// steps i = 1..nZ-1: Re x' = p.re Re x + p.im Im x, Im x' = q.re Re x + q.im Im x.
 template <class M>
 __global__ void k_step_table(const cplx* __restrict__ coef, const cplx* __restrict__ cconst,
                              const cplx* __restrict__ pump, int N, int nZ, double scale, cplx* __restrict__ tab) {
@@ -45,10 +48,8 @@ __global__ void k_step_table(const cplx* __restrict__ coef, const cplx* __restri
   const cplx P2 = pump[(size_t)(2 * i) * N + k];
   const cplx r1 = rk4_step<M>(c, cconst, mk(1.0, 0.0), P0, P1, P2);
   const cplx ri = rk4_step<M>(c, cconst, mk(0.0, 1.0), P0, P1, P2);
-  const cplx iri = mk(-ri.im, ri.re);  // i * R(i)
-  const cplx al = 0.5 * (r1 - iri), be = 0.5 * (r1 + iri);
-  tab[((size_t)i * N + k) * 2] = scale * al;
-  tab[((size_t)i * N + k) * 2 + 1] = scale * be;
+  tab[((size_t)i * N + k) * 2] = mk(scale * r1.re, scale * ri.re);
+  tab[((size_t)i * N + k) * 2 + 1] = mk(scale * r1.im, scale * ri.im);
 }
 
 template <int L>
@@ -107,12 +108,11 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
   }
 
   for (int i = 1; i < kp.nZ; i++) {
-    // --- RK4 nonlinear step as its R-linear propagator
+    // --- RK4 nonlinear step as its R-linear propagator (a real 2 x 2 matrix)
 #pragma unroll
     for (int m = 0; m < 8; m++) {
       const cplx x = a[0][m];
-      a[0][m] = mk(al[m].re * x.re - al[m].im * x.im + be[m].re * x.re + be[m].im * x.im,
-                   al[m].re * x.im + al[m].im * x.re + be[m].im * x.re - be[m].re * x.im);
+      a[0][m] = mk(al[m].re * x.re + al[m].im * x.im, be[m].re * x.re + be[m].im * x.im);
     }
     if (i + 1 < kp.nZ) load_prop_row<L>(al, be, kp.tab + (size_t)(i + 1) * N * 2, t);
 
