@@ -81,7 +81,6 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
   __shared__ cplx lds[FPB * F::lds_elems];
   __shared__ cplx twl[PropTw<L>::LDS_ELEMS];
   fill_twiddle_lds<L>(twl, kp.tw);
-  constexpr bool LEAN = PropTw<L>::LEAN;
 
   const int lf = threadIdx.x / T, t = threadIdx.x % T;
   const int field = blockIdx.x * FPB + lf;
@@ -90,8 +89,17 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
 
   typename PropTw<L>::type tw;
   init_twiddles<L>(tw, kp.tw, twl, t);
-  cplx ds[LEAN ? 1 : 8];  // from N = 4096 read from the cached row at each use
-  if constexpr (!LEAN) {
+  // Where the dispersion row and the next step's propagator rows live.  N = 512
+  // (one wave per field and SIMD): the dispersion row is read from L1/L2 where it
+  // is used and the propagator rows are fetched after the forward transform, so
+  // neither is held in registers across a whole step; held across the step they
+  // spilled to AGPRs (92 register moves per step, 3 % of the step time).  Other
+  // sizes keep the dispersion row in registers below N = 4096 and fetch the rows
+  // a whole step ahead (measured at N = 1024: the late fetch is 6 % slower).
+  constexpr bool LATE = L == 9;
+  constexpr bool DS_MEM = LATE || PropTw<L>::LEAN;
+  cplx ds[DS_MEM ? 1 : 8];
+  if constexpr (!DS_MEM) {
 #pragma unroll
     for (int m = 0; m < 8; m++) ds[m] = kp.dstep[t + T * m];
   }
@@ -114,12 +122,13 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
       const cplx x = a[0][m];
       a[0][m] = mk(al[m].re * x.re + al[m].im * x.im, be[m].re * x.re + be[m].im * x.im);
     }
-    if (i + 1 < kp.nZ) load_prop_row<L>(al, be, kp.tab + (size_t)(i + 1) * N * 2, t);
+    if (!LATE && i + 1 < kp.nZ) load_prop_row<L>(al, be, kp.tab + (size_t)(i + 1) * N * 2, t);
 
     // --- dispersion step (src/_NonlinearMedium.hpp:212-215)
     fft_field1<L, false>(a[0], buf, t, tw);
 #pragma unroll
-    for (int m = 0; m < 8; m++) a[0][m] = a[0][m] * (LEAN ? kp.dstep[t + T * m] : ds[LEAN ? 0 : m]);
+    for (int m = 0; m < 8; m++) a[0][m] = a[0][m] * (DS_MEM ? kp.dstep[t + T * m] : ds[DS_MEM ? 0 : m]);
+    if (LATE && i + 1 < kp.nZ) load_prop_row<L>(al, be, kp.tab + (size_t)(i + 1) * N * 2, t);
     if (i < kp.nZ - 1) {
       if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, 1, 0, i, N), a[0], t, active);
       fft_field1<L, true>(a[0], buf, t, tw);
