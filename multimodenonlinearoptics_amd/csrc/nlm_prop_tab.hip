@@ -89,15 +89,17 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
 
   typename PropTw<L>::type tw;
   init_twiddles<L>(tw, kp.tw, twl, t);
-  // Where the dispersion row and the next step's propagator rows live.  N = 512
-  // (one wave per field and SIMD): the dispersion row is read from L1/L2 where it
-  // is used and the propagator rows are fetched after the forward transform, so
-  // neither is held in registers across a whole step; held across the step they
-  // spilled to AGPRs (92 register moves per step, 3 % of the step time).  Other
-  // sizes keep the dispersion row in registers below N = 4096 and fetch the rows
-  // a whole step ahead (measured at N = 1024: the late fetch is 6 % slower).
-  constexpr bool LATE = L == 9;
-  constexpr bool DS_MEM = LATE || PropTw<L>::LEAN;
+  // Where the dispersion row and the next step's propagator rows live (measured
+  // per size on Green's launches).  N = 512 (one wave per field and SIMD): the
+  // dispersion row is read from L1/L2 where it is used and the propagator rows are
+  // fetched after the forward transform, so neither is held in registers across a
+  // whole step; held across the step they spilled to AGPRs (92 register moves per
+  // step, 3 % of the step time).  N = 1024: the late fetch alone is 7 % faster,
+  // the dispersion row read at use too is slower.  N = 2048 is fastest with both
+  // held; from N = 4096 the dispersion row is read at use (PropTw::LEAN).
+  constexpr int LOG = FftShape<L>::LOG;
+  constexpr bool LATE = LOG == 9 || LOG == 10;
+  constexpr bool DS_MEM = LOG == 9 || PropTw<L>::LEAN;
   cplx ds[DS_MEM ? 1 : 8];
   if constexpr (!DS_MEM) {
 #pragma unroll
