@@ -248,6 +248,10 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
   // latency hides behind the transforms (at the point of use from N = 4096)
   // (with RB false this step's rows are loaded at the top of the step instead)
   constexpr int PR = LEAN || !RB ? 1 : 8;
+  // LATE (every state kept, full registers): this step's rows and the next
+  // state are fetched after the forward transform of g, so neither is held in
+  // registers across it (their latency hides behind the inverse transform)
+  constexpr bool LATE = !LEAN && !RB;
   cplx al[8], be[8], pa[PR], pb[PR];
   if constexpr (!LEAN && RB) {
     prop_row<L>(al, be, ap.tab + (size_t)ap.iHi * N * 2, t);
@@ -259,7 +263,6 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
     // with FUSE the state's dispersion step is undone in the same transforms:
     // xr = IFFT(conj(exp(i dz D)) FFT(a_{i-1})), from which a_{i-2} = R_{i-1}^-1(xr)
     const bool rebuild = RB && i >= 2 && (i - 2) % ap.K != 0;
-    if constexpr (!LEAN && !RB) prop_row<L>(al, be, ap.tab + (size_t)i * N * 2, t);
     const bool fused = FUSE && rebuild && !gFreq;
     cplx xr[8];
     if (fused) {
@@ -281,6 +284,14 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
       }
 #pragma unroll
       for (int m = 0; m < 8; m++) g[m] = (LEAN ? conj(ap.dstep[t + T * m]) * (1.0 / N) : dsr[LEAN ? 0 : m]) * g[m];
+      if constexpr (LATE) {
+        prop_row<L>(al, be, ap.tab + (size_t)i * N * 2, t);
+        // state a_{i-1} of this step (kept checkpoint; the first step's is set above)
+        if (i != ap.iHi) {
+#pragma unroll
+          for (int m = 0; m < 8; m++) x[m] = ap.ckpt[((size_t)fld * ap.nSeg + (i - 1) / ap.K) * N + t + T * m];
+        }
+      }
       fft_field1<L, true>(g, buf, t, tw);  // unscaled inverse
     }
     gFreq = false;
@@ -303,7 +314,7 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
     }
 
     // state a_{i-2} for the next (earlier) step
-    if (i >= 2) {
+    if (!LATE && i >= 2) {
       if (!rebuild) {
 #pragma unroll
         for (int m = 0; m < 8; m++) x[m] = ap.ckpt[((size_t)fld * ap.nSeg + (i - 2) / ap.K) * N + t + T * m];
@@ -332,6 +343,10 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
         if (i >= 3) prop_row<L>(pa, pb, ap.tab + (size_t)(i - 2) * N * 2, t);
       }
     }
+  }
+  if (LATE && ap.iLo > 0 && ap.iLo < ap.iHi) {
+#pragma unroll
+    for (int m = 0; m < 8; m++) x[m] = ap.ckpt[((size_t)fld * ap.nSeg + (ap.iLo - 1) / ap.K) * N + t + T * m];
   }
   if (ap.iLo > 0 && active) {
     cplx* st = ap.state + (size_t)fld * 2 * N;
