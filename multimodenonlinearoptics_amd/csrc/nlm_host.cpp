@@ -1343,7 +1343,7 @@ Grid Cascade::batchSignalSimulation(const Grid& inputProfs, bool inTimeDomain, b
 // ---------------------------------------------------------------------------
 
 MultiBatch::MultiBatch(const std::vector<Medium*>& media, const std::vector<std::vector<cd>>& inputs,
-                       bool inTimeDomain, bool autotune)
+                       bool inTimeDomain, bool autotune, bool stageRows)
     : media_(media) {
   if (media.size() != inputs.size()) throw std::invalid_argument("one input row per medium is required");
   const size_t n = media.size();
@@ -1382,7 +1382,7 @@ MultiBatch::MultiBatch(const std::vector<Medium*>& media, const std::vector<std:
     }
     outLen_.push_back(md->nS_ * md->N_);
   }
-  check(nlm_multi_create((int)n, items.data(), autotune ? 1 : 0, &m_));
+  check(nlm_multi_create((int)n, items.data(), (autotune ? 1 : 0) | (stageRows ? 0 : 2), &m_));
 }
 
 MultiBatch::~MultiBatch() {

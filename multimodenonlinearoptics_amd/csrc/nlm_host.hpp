@@ -274,8 +274,10 @@ class Chi3GNLSE : public FullyNonlinearMedium {
 // runPump=True) of that medium (src/_NonlinearMedium.cpp:378-444).
 class MultiBatch {
  public:
+  // stageRows: nFreqs 4096 media with a generated pump stage their pump rows in a
+  // device buffer at the start of each launch (see nlm_multi_create)
   MultiBatch(const std::vector<Medium*>& media, const std::vector<std::vector<cd>>& inputs, bool inTimeDomain,
-             bool autotune);
+             bool autotune, bool stageRows = true);
   ~MultiBatch();
   MultiBatch(const MultiBatch&) = delete;
   MultiBatch& operator=(const MultiBatch&) = delete;

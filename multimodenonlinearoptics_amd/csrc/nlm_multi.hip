@@ -33,6 +33,72 @@ __global__ void k_zero_ints(int* p, int n) {
   if (k < n) p[k] = 0;
 }
 
+// Staged pump rows.  From N = 4096 a field's 512 lanes have 256 registers each,
+// and a propagation that generates its pump rows (InlinePump: the envelope's
+// spectrum and the running phases held per lane, two inverse transforms per
+// step) spills to scratch.  Such a class instead starts with this kernel, which
+// writes every instance's time-domain pump rows (src/_NonlinearMedium.cpp:
+// 225-250: row 0 the envelope, row r > 0 IFFT(FFT(envelope) exp(i r dzp D_p)),
+// each times the beam-profile factor) into a buffer that the class's
+// propagation kernel then reads like a medium's pump grid.  One workgroup per
+// chunk of ROWS_CHUNK rows of one instance (chunks[2b] = instance, chunks[2b+1]
+// = first row); every workgroup transforms the envelope once.
+constexpr int ROWS_L = 12;        // sizes that stage their pump rows
+constexpr int ROWS_CHUNK = 32;    // rows per workgroup
+
+template <int L>
+__global__ void __launch_bounds__(FftShape<L>::T)
+    k_multi_rows(const MultiInst* __restrict__ inst, const int* __restrict__ chunks, const cplx* __restrict__ twt) {
+  using F = FftShape<L>;
+  constexpr int N = F::N, T = F::T;
+  __shared__ cplx lds[F::lds_elems];
+  __shared__ cplx twl[PropTw<L>::LDS_ELEMS];
+  fill_twiddle_lds<L>(twl, twt);
+  const int t = threadIdx.x;
+  const MultiInst& I = inst[chunks[2 * blockIdx.x]];
+  const int r0 = chunks[2 * blockIdx.x + 1];
+  const int nZP = 2 * I.nZ - 1;
+  const int r1 = r0 + ROWS_CHUNK < nZP ? r0 + ROWS_CHUNK : nZP;
+  cplx* rows = const_cast<cplx*>(I.pump0);
+  typename PropTw<L>::type tw;
+  init_twiddles<L>(tw, twt, twl, t);
+  cplx f0[8];
+#pragma unroll
+  for (int m = 0; m < 8; m++) f0[m] = I.env[t + T * m];
+  if (r0 == 0) {
+    const double s0 = I.strength ? I.strength[0] : 1.0;
+#pragma unroll
+    for (int m = 0; m < 8; m++) rows[t + T * m] = s0 * f0[m];
+  }
+  fft_field1<L, false>(f0, lds, t, tw);
+  // the phase exp(i r dzp D_p): exact at the chunk's first row, then advanced by
+  // exp(i dzp D_p) per row (as the in-kernel generator does between its exact
+  // re-evaluations)
+  const int rb = r0 > 0 ? r0 : 1;
+  cplx ph[8], e1[8];
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    const double d = I.Dp[t + T * m];
+    double sn, cs;
+    sincos_reduced((double)rb * I.dzp * d, &sn, &cs);
+    ph[m] = mk(cs, sn);
+    sincos_reduced(I.dzp * d, &sn, &cs);
+    e1[m] = mk(cs, sn);
+  }
+  for (int r = rb; r < r1; r++) {
+    cplx f[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      if (r > rb) ph[m] = ph[m] * e1[m];
+      f[m] = f0[m] * ph[m];
+    }
+    fft_field1<L, true>(f, lds, t, tw);
+    const double sc = (I.strength ? I.strength[r] : 1.0) * (1.0 / N);
+#pragma unroll
+    for (int m = 0; m < 8; m++) rows[(size_t)r * N + t + T * m] = f[m] * sc;
+  }
+}
+
 // Kernel of a class: models sharing a right-hand side share kernels (Chi2AFC
 // runs Chi2SFG's, Chi2ASHG Chi2SHG's).
 static int class_model(int model) {
@@ -60,7 +126,9 @@ static multi_fn pick_multi(int model, int L, bool inl) {
 
 struct MultiClass {
   int model, L, count, grid;
-  bool inl;  // pump rows generated in the kernel
+  bool inl;   // pump rows generated in the kernel
+  bool rows;  // pump rows staged by k_multi_rows before the propagation kernel
+  int chunkOff = 0, nChunks = 0;
   multi_fn fn;
   size_t orderOff;  // offset of this class's instance order in the order array
   hipStream_t stream = nullptr;
@@ -75,6 +143,8 @@ struct nlm_multi_s {
   DevBuf<double> realBuf;
   DevBuf<MultiInst> inst;
   DevBuf<int> order, counters;
+  DevBuf<cplx> rowsBuf;  // staged pump rows of the classes with MultiClass::rows
+  DevBuf<int> chunks;    // their generation chunks (instance, first row)
   std::vector<MultiClass> classes;
   hipStream_t stream = nullptr;
   hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
@@ -92,7 +162,14 @@ static int device_cus() {
   return cus > 0 ? cus : 256;
 }
 
+static void launch_rows(nlm_multi_s* m, const MultiClass& c, const cplx* tw, hipStream_t s) {
+  if (c.rows && c.nChunks > 0)
+    hipLaunchKernelGGL((k_multi_rows<ROWS_L>), dim3(c.nChunks), dim3(FftShape<ROWS_L>::T), 0, s, m->inst.p,
+                       m->chunks.p + 2 * (size_t)c.chunkOff, tw);
+}
+
 static int run_class(nlm_multi_s* m, const MultiClass& c, int grid, const cplx* tw, hipStream_t s) {
+  launch_rows(m, c, tw, s);
   hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, m->counters.p + (&c - m->classes.data()), 1);
   c.fn(m->inst.p, m->order.p + c.orderOff, c.count, m->counters.p + (&c - m->classes.data()), tw, grid, s);
   HIP_TRY(hipGetLastError());
@@ -119,6 +196,8 @@ extern "C" int nlm_multi_destroy(nlm_multi* m) {
 extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune, nlm_multi** out) {
   if (!out || n < 0 || (n > 0 && !items)) return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
   *out = nullptr;
+  const bool noStage = (autotune & 2) != 0;
+  autotune &= 1;
   nlm_multi_s* m = new nlm_multi_s();
   auto fail = [&](int code) {
     nlm_multi_destroy(m);
@@ -173,8 +252,7 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
   }
   int e;
   if ((e = m->inBuf.ensure(inTot)) || (e = m->outBuf.ensure(outTot)) || (e = m->envBuf.ensure(envTot)) ||
-      (e = m->realBuf.ensure(realTot)) || (e = m->inst.ensure(n)) || (e = m->order.ensure(n)) ||
-      (e = m->counters.ensure(byClass.size() + 1)))
+      (e = m->realBuf.ensure(realTot)) || (e = m->inst.ensure(n)) || (e = m->order.ensure(n)))
     return fail(e);
 
   // descriptors and packed uploads
@@ -224,6 +302,34 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
       d.dzp = it.dzp;
     }
   }
+  // classes that stage their pump rows: ROWS_L with a generated pump (key 2),
+  // when all their rows fit a quarter of the device memory and half of what is
+  // free.  (Staging only the longer half of such a class, the shorter half
+  // generating its rows in the kernel from the start, was slower: 211 against
+  // 197 ms on 2000 SHG / PDC media.)
+  if (!noStage) {
+    std::vector<std::tuple<int, int, int>> keys;
+    for (auto& kv : byClass)
+      if (std::get<2>(kv.first) == 1 && std::get<1>(kv.first) == ROWS_L) keys.push_back(kv.first);
+    for (const auto& k : keys) {
+      byClass[std::make_tuple(std::get<0>(k), ROWS_L, 2)] = byClass[k];
+      byClass.erase(k);
+    }
+  }
+  if ((e = m->counters.ensure(byClass.size() + 1))) return fail(e);
+  size_t rowsTot = 0;
+  for (auto& kv : byClass)
+    if (std::get<2>(kv.first) == 2)
+      for (int i : kv.second) rowsTot += (2 * (size_t)hi[i].nZ - 1) * m->N[i];
+  bool useRows = rowsTot > 0 && !noStage;
+  if (useRows) {
+    size_t freeB = 0, totalB = 0;
+    useRows = hipMemGetInfo(&freeB, &totalB) == hipSuccess && rowsTot * sizeof(cplx) <= totalB / 4 &&
+              rowsTot * sizeof(cplx) <= freeB / 2;
+    if (useRows && (e = m->rowsBuf.ensure(rowsTot))) return fail(e);
+  }
+  std::vector<int> hchunks;
+  size_t rowsOff = 0;
   // classes and their longest-first orders
   std::vector<int> hord;
   hord.reserve(n);
@@ -232,7 +338,21 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
     c.model = std::get<0>(kv.first);
     c.L = std::get<1>(kv.first);
     c.inl = std::get<2>(kv.first) != 0;
-    c.fn = pick_multi(c.model, c.L, c.inl);
+    c.rows = useRows && std::get<2>(kv.first) == 2;
+    c.fn = pick_multi(c.model, c.L, c.inl && !c.rows);
+    if (c.rows) {
+      c.chunkOff = (int)(hchunks.size() / 2);
+      for (int i : kv.second) {
+        hi[i].pump0 = m->rowsBuf.p + rowsOff;
+        const int nZP = 2 * hi[i].nZ - 1;
+        rowsOff += (size_t)nZP * m->N[i];
+        for (int r = 0; r < nZP; r += ROWS_CHUNK) {
+          hchunks.push_back(i);
+          hchunks.push_back(r);
+        }
+      }
+      c.nChunks = (int)(hchunks.size() / 2) - c.chunkOff;
+    }
     std::vector<int> idx = kv.second;
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return hi[a].nZ > hi[b].nZ; });
     c.orderOff = hord.size();
@@ -240,6 +360,10 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
     c.count = (int)idx.size();
     c.grid = 1;
     m->classes.push_back(c);
+  }
+  if (!hchunks.empty()) {
+    if ((e = m->chunks.ensure(hchunks.size()))) return fail(e);
+    HIP_TRY(hipMemcpy(m->chunks.p, hchunks.data(), sizeof(int) * hchunks.size(), hipMemcpyHostToDevice));
   }
   if (n > 0) {
     HIP_TRY(hipMemcpy(m->inst.p, hi.data(), sizeof(MultiInst) * n, hipMemcpyHostToDevice));
@@ -257,7 +381,7 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
     if ((e = get_twiddles(1 << c.L, &tw))) return fail(e);
     const int cand[4] = {cus, 2 * cus, 4 * cus, 8 * cus};
     int best = std::min(c.count, 4 * cus);
-    const auto key = std::make_tuple(c.model, c.L, c.inl ? 1 : 0, c.count);
+    const auto key = std::make_tuple(c.model, c.L, c.rows ? 2 : c.inl ? 1 : 0, c.count);
     bool cached = false;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -302,6 +426,7 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
       (void)get_twiddles(1 << c.L, &tw);
       if (ce == hipSuccess) ce = hipStreamWaitEvent(c.stream, m->fork, 0);
       if (ce == hipSuccess) {
+        launch_rows(m, c, tw, c.stream);
         hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c.stream, m->counters.p + (&c - m->classes.data()), 1);
         c.fn(m->inst.p, m->order.p + c.orderOff, c.count, m->counters.p + (&c - m->classes.data()), tw, c.grid,
              c.stream);

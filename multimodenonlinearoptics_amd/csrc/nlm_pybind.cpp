@@ -378,14 +378,16 @@ PYBIND11_MODULE(nonlinearmedium, m) {
                               "One input row per medium, for many media of different models, sizes and parameters, "
                               "propagated together by one graph launch (extension).  output(i) equals "
                               "media[i].batchSignalSimulation(inputs[i][None], inTimeDomain)[0].")
-      .def(py::init([](const py::list& media, const py::list& inputs, bool inTimeDomain, bool autotune) {
+      .def(py::init([](const py::list& media, const py::list& inputs, bool inTimeDomain, bool autotune,
+                       bool stagePumpRows) {
              std::vector<Medium*> v;
              for (auto h : media) v.push_back(&h.cast<Medium&>());
              std::vector<std::vector<cd>> in;
              for (auto h : inputs) in.push_back(to_vec(h.cast<carr>()));
-             return new nlm::MultiBatch(v, in, inTimeDomain, autotune);
+             return new nlm::MultiBatch(v, in, inTimeDomain, autotune, stagePumpRows);
            }),
-           "media"_a, "inputs"_a, "inTimeDomain"_a = false, "autotune"_a = true, py::keep_alive<1, 2>())
+           "media"_a, "inputs"_a, "inTimeDomain"_a = false, "autotune"_a = true, "stagePumpRows"_a = true,
+           py::keep_alive<1, 2>())
       .def("launch", [](nlm::MultiBatch& b) {
         py::gil_scoped_release nogil;
         b.launch();

@@ -53,11 +53,13 @@ def test_mixed_batch_frequency_domain_and_sizes(nlm, oracle):
         assert rel_l2(b.output(i), exp) < TOL, i
 
 
-def test_large_pdc_class_matches_oracle(nlm, oracle):
-    """Chi2PDC at N = 4096 (512 lanes per field, pump rows generated in the
-    kernel) with instances of 3, 70 and 150 steps (the generator's phase is
-    re-evaluated exactly every 64 rows) and a Chi2SHG medium of the same size in
-    the same batch."""
+@pytest.mark.parametrize("stage", [True, False])
+def test_large_pdc_class_matches_oracle(nlm, oracle, stage):
+    """Chi2PDC at N = 4096 (512 lanes per field) with instances of 3, 70 and 150
+    steps and a Chi2SHG medium of the same size in the same batch: the pump rows
+    staged in a device buffer at the start of the launch (chunks of 32 rows, the
+    last ones partial), or generated in the kernel (the phase re-evaluated
+    exactly every 64 rows)."""
     r = np.random.default_rng(41)
     kws = [dict(relativeLength=1.0, nlLength=0.5 + 0.3 * j, beta2=0.2 * j - 0.1, beta2s=-0.3, beta1=0.1 * j,
                 diffBeta0=0.5 - j, tMax=20, tPrecision=4096, nZSteps=nz) for j, nz in enumerate((3, 70, 150))]
@@ -68,10 +70,10 @@ def test_large_pdc_class_matches_oracle(nlm, oracle):
     ref = [getattr(oracle, n)(**kw) for n, kw in insts]
     cols = [kw["tPrecision"] for _, kw in insts]
     xs = [r.standard_normal(c) + 1j * r.standard_normal(c) for c in cols]
-    b = nlm.MultiBatch(media, xs, inTimeDomain=True, autotune=False)
+    b = nlm.MultiBatch(media, xs, inTimeDomain=True, autotune=False, stagePumpRows=stage)
     b.run()
     for i, (rm, x) in enumerate(zip(ref, xs)):
-        print("large-N instance", i, insts[i][0], insts[i][1]["nZSteps"], flush=True)
+        print("large-N instance", i, insts[i][0], insts[i][1]["nZSteps"], "staged", stage, flush=True)
         exp = rm.batchSignalSimulation(x[None, :], inTimeDomain=True)[0]
         assert rel_l2(b.output(i), exp) < TOL, (i, rel_l2(b.output(i), exp))
     first = [b.output(i) for i in range(len(b))]
