@@ -236,6 +236,15 @@ struct InlinePump {
   }
 };
 
+__device__ __forceinline__ int opaque_lane(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+template <int L>
+__device__ __forceinline__ void retarget(TableTwiddles<L>& tw, int t) { tw.t = t; }
+template <int L>
+__device__ __forceinline__ void retarget(RegTwiddles<L>&, int) {}
+
 // One propagation (the loop of signalSimulationTemplate) of field `field` on
 // the T lanes that hold it; INLINE generates the pump rows instead of reading
 // the pump grid.
@@ -297,7 +306,16 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
     }
   }
 
+  const int t0 = t;
   for (int i = 1; i < kp.nZ; i++) {
+    // The lane index as a value the compiler cannot see through: the LDS,
+    // twiddle and memory offsets derived from it are formed inside the step
+    // instead of being hoisted out of the z loop, where they took up to ~120
+    // registers that spilled (measured on the batch kernels: Chi2SHG at
+    // N = 4096 115 -> 78 registers spilled to scratch, N = 1024 / 2048 none
+    // instead of 2 / 6; Chi2PDC reading its pump rows at N = 4096 48 -> 17).
+    const int t = opaque_lane(t0);
+    retarget(tw, t);
     // --- RK4 nonlinear step (DiffEq + update, src/_NonlinearMedium.hpp:207-210)
     const cplx* c = kp.coef + (size_t)i * 3 * NC;
 #pragma unroll
