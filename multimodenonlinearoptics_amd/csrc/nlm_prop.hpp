@@ -346,14 +346,12 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
     for (int r = 0; r < NP; r++) {
 #pragma unroll
       for (int m = 0; m < 8; m++) pp[r][m] = pc[r][m];
-      if (i + 1 < kp.nZ) {
-        if constexpr (INLINE) {
-          gen.next(kp, buf, t, tw, pi[r]);
-          gen.next(kp, buf, t, tw, pc[r]);
-        } else {
-          load_pump<L>(pi[r], pumps[r] + (size_t)(2 * i + 1) * N, t);
-          load_pump<L>(pc[r], pumps[r] + (size_t)(2 * i + 2) * N, t);
-        }
+      // pump grids: the next rows are loaded here so that their latency hides
+      // behind the dispersion transforms; generated rows are made after them
+      // (below), so that the generator's rows are not held across the transforms
+      if (!INLINE && i + 1 < kp.nZ) {
+        load_pump<L>(pi[r], pumps[r] + (size_t)(2 * i + 1) * N, t);
+        load_pump<L>(pc[r], pumps[r] + (size_t)(2 * i + 2) * N, t);
       }
     }
 
@@ -371,6 +369,12 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
         if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, NS, q, i, N), a[q], t, active);
         if (kp.ckpt && i % kp.ckptEvery == 0)
           store_row<L>(kp.ckpt + (((size_t)field * kp.nSeg + i / kp.ckptEvery) * NS + q) * N, a[q], t, active);
+      }
+    }
+    if constexpr (INLINE) {
+      if (i + 1 < kp.nZ) {
+        gen.next(kp, buf, t, tw, pi[0]);
+        gen.next(kp, buf, t, tw, pc[0]);
       }
     }
   }
