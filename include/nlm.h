@@ -203,11 +203,10 @@ typedef struct {
 } nlm_multi_item;
 typedef struct nlm_multi_s nlm_multi;
 /* autotune (flags): bit 0 time each class alone for a few persistent grid sizes
- * and keep the fastest (once per process and class shape); bit 1 keep generated
- * pumps in the kernel at every size.  Otherwise nFreqs 4096 classes with a
- * generated pump stage their time-domain pump rows in a device buffer at the
- * start of each launch (a quarter of the device memory at most) and read them
- * like a pump grid. */
+ * and keep the fastest (once per process and class shape); bit 1 stage the
+ * pump rows of nFreqs 4096 classes with a generated pump: each launch first
+ * writes their time-domain rows to a device buffer (a quarter of the device
+ * memory at most), which the class's kernel then reads like a pump grid. */
 int nlm_multi_create(int n, const nlm_multi_item* items, int autotune, nlm_multi** out);
 int nlm_multi_launch(nlm_multi* batch);             /* one graph launch, asynchronous */
 int nlm_multi_sync(nlm_multi* batch, double* ms);   /* wait; ms = device time of the last launch */

@@ -1382,7 +1382,7 @@ MultiBatch::MultiBatch(const std::vector<Medium*>& media, const std::vector<std:
     }
     outLen_.push_back(md->nS_ * md->N_);
   }
-  check(nlm_multi_create((int)n, items.data(), (autotune ? 1 : 0) | (stageRows ? 0 : 2), &m_));
+  check(nlm_multi_create((int)n, items.data(), (autotune ? 1 : 0) | (stageRows ? 2 : 0), &m_));
 }
 
 MultiBatch::~MultiBatch() {

@@ -277,7 +277,7 @@ class MultiBatch {
   // stageRows: nFreqs 4096 media with a generated pump stage their pump rows in a
   // device buffer at the start of each launch (see nlm_multi_create)
   MultiBatch(const std::vector<Medium*>& media, const std::vector<std::vector<cd>>& inputs, bool inTimeDomain,
-             bool autotune, bool stageRows = true);
+             bool autotune, bool stageRows = false);
   ~MultiBatch();
   MultiBatch(const MultiBatch&) = delete;
   MultiBatch& operator=(const MultiBatch&) = delete;

@@ -196,7 +196,7 @@ extern "C" int nlm_multi_destroy(nlm_multi* m) {
 extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune, nlm_multi** out) {
   if (!out || n < 0 || (n > 0 && !items)) return set_err(NLM_INVALID_ARGUMENT, "invalid arguments");
   *out = nullptr;
-  const bool noStage = (autotune & 2) != 0;
+  const bool stage = (autotune & 2) != 0;
   autotune &= 1;
   nlm_multi_s* m = new nlm_multi_s();
   auto fail = [&](int code) {
@@ -307,7 +307,7 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
   // free.  (Staging only the longer half of such a class, the shorter half
   // generating its rows in the kernel from the start, was slower: 211 against
   // 197 ms on 2000 SHG / PDC media.)
-  if (!noStage) {
+  if (stage) {
     std::vector<std::tuple<int, int, int>> keys;
     for (auto& kv : byClass)
       if (std::get<2>(kv.first) == 1 && std::get<1>(kv.first) == ROWS_L) keys.push_back(kv.first);
@@ -321,7 +321,7 @@ extern "C" int nlm_multi_create(int n, const nlm_multi_item* items, int autotune
   for (auto& kv : byClass)
     if (std::get<2>(kv.first) == 2)
       for (int i : kv.second) rowsTot += (2 * (size_t)hi[i].nZ - 1) * m->N[i];
-  bool useRows = rowsTot > 0 && !noStage;
+  bool useRows = rowsTot > 0;
   if (useRows) {
     size_t freeB = 0, totalB = 0;
     useRows = hipMemGetInfo(&freeB, &totalB) == hipSuccess && rowsTot * sizeof(cplx) <= totalB / 4 &&

@@ -386,7 +386,7 @@ PYBIND11_MODULE(nonlinearmedium, m) {
              for (auto h : inputs) in.push_back(to_vec(h.cast<carr>()));
              return new nlm::MultiBatch(v, in, inTimeDomain, autotune, stagePumpRows);
            }),
-           "media"_a, "inputs"_a, "inTimeDomain"_a = false, "autotune"_a = true, "stagePumpRows"_a = true,
+           "media"_a, "inputs"_a, "inTimeDomain"_a = false, "autotune"_a = true, "stagePumpRows"_a = false,
            py::keep_alive<1, 2>())
       .def("launch", [](nlm::MultiBatch& b) {
         py::gil_scoped_release nogil;
