@@ -258,7 +258,16 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
     if (ap.iHi >= 2) prop_row<L>(pa, pb, ap.tab + (size_t)(ap.iHi - 1) * N * 2, t);
   }
 
+  // N = 2048 and 4096 (two waves per SIMD): the lane index is opaque to the
+  // compiler inside each step, so the offsets derived from it are not hoisted out
+  // of the loop into registers (the rebuilding sweep spilled 38 registers to
+  // scratch at N = 2048 and 16 at N = 4096 with them hoisted, 2 without; at
+  // N = 8192, four waves per SIMD, it spills more, so it keeps them hoisted)
+  constexpr bool OPQ = LEAN && L <= 12;
+  const int t0 = t;
   for (int i = ap.iHi; i > ap.iLo; i--) {
+    const int t = OPQ ? opaque_lane(t0) : t0;
+    if constexpr (OPQ) retarget(tw, t);
     // adjoint of the dispersion step: h = g_r = IFFT(conj(exp(i dz D)) FFT(g));
     // with FUSE the state's dispersion step is undone in the same transforms:
     // xr = IFFT(conj(exp(i dz D)) FFT(a_{i-1})), from which a_{i-2} = R_{i-1}^-1(xr)
