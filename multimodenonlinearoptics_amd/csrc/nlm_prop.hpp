@@ -236,6 +236,11 @@ struct InlinePump {
   }
 };
 
+// Read-only data that every lane of a workgroup reads at the same address, in the
+// constant address space so that the compiler loads it with scalar loads.
+typedef const __attribute__((address_space(4))) cplx uniform_cplx;
+__device__ __forceinline__ const uniform_cplx* to_uniform(const cplx* p) { return (const uniform_cplx*)p; }
+
 __device__ __forceinline__ int opaque_lane(int x) {
   asm volatile("" : "+v"(x));
   return x;
@@ -306,6 +311,14 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
     }
   }
 
+  // the medium's constants (Chi3: the nonlinear step), uniform like the coefficients
+  constexpr int NCC = M::NCC > 0 ? M::NCC : 1;
+  cplx cc[NCC];
+  {
+    const uniform_cplx* ccq = to_uniform(kp.cconst);
+#pragma unroll
+    for (int k = 0; k < M::NCC; k++) cc[k] = mk(ccq[k].re, ccq[k].im);
+  }
   const int t0 = t;
   for (int i = 1; i < kp.nZ; i++) {
     // The lane index as a value the compiler cannot see through: the LDS,
@@ -317,7 +330,15 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
     const int t = opaque_lane(t0);
     retarget(tw, t);
     // --- RK4 nonlinear step (DiffEq + update, src/_NonlinearMedium.hpp:207-210)
-    const cplx* c = kp.coef + (size_t)i * 3 * NC;
+    // The step's coefficients are the same for every lane: read through the
+    // constant address space they are scalar loads into SGPRs, instead of vector
+    // loads whose latency each RK stage waited for.
+    cplx c[3 * NC > 0 ? 3 * NC : 1];
+    {
+      const uniform_cplx* cq = to_uniform(kp.coef + (size_t)i * 3 * NC);
+#pragma unroll
+      for (int k = 0; k < 3 * NC; k++) c[k] = mk(cq[k].re, cq[k].im);
+    }
 #pragma unroll
     for (int m = 0; m < 8; m++) {
       cplx x[NS], y[NS], k1[NS], k2[NS], k3[NS], k4[NS], P0[NPP], P1[NPP], P2[NPP];
@@ -329,16 +350,16 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
       }
 #pragma unroll
       for (int q = 0; q < NS; q++) x[q] = a[q][m];
-      M::template f<0>(c, kp.cconst, x, P0, k1);
+      M::template f<0>(c, cc, x, P0, k1);
 #pragma unroll
       for (int q = 0; q < NS; q++) y[q] = x[q] + 0.5 * k1[q];
-      M::template f<1>(c + NC, kp.cconst, y, P1, k2);
+      M::template f<1>(c + NC, cc, y, P1, k2);
 #pragma unroll
       for (int q = 0; q < NS; q++) y[q] = x[q] + 0.5 * k2[q];
-      M::template f<2>(c + NC, kp.cconst, y, P1, k3);
+      M::template f<2>(c + NC, cc, y, P1, k3);
 #pragma unroll
       for (int q = 0; q < NS; q++) y[q] = x[q] + k3[q];
-      M::template f<3>(c + 2 * NC, kp.cconst, y, P2, k4);
+      M::template f<3>(c + 2 * NC, cc, y, P2, k4);
 #pragma unroll
       for (int q = 0; q < NS; q++) a[q][m] = x[q] + (((k1[q] + 2.0 * k2[q]) + 2.0 * k3[q]) + k4[q]) * (1. / 6.);
     }
