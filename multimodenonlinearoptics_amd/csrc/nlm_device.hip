@@ -764,13 +764,14 @@ int tune_variant(nlm_plan* p, const KParams& kp, int grid, int* best) {
     HIP_TRY(hipEventCreate(&g_tune_ev[1]));
   }
   // the first pass warms the clocks and the code objects; a variant scores the
-  // slowest of its three timed launches, so that one whose time varies from
-  // launch to launch is not chosen on a fast sample (measured at N = 512: the
-  // padding variant 1 took 2.03 or 2.25 ms from one launch to the next, variant 0
-  // 2.02-2.07 ms)
-  constexpr int PASSES = 4;
+  // slowest of its timed launches, so that one whose time varies from launch to
+  // launch is not chosen on fast samples (measured at N = 512: the padding
+  // variant 1 took 2.03 or 2.25 ms from one launch to the next, variant 0
+  // 2.02-2.07 ms).  As many timed passes as fit ~60 ms, 2 to 6.
   float worst[4] = {0, 0, 0, 0};
-  for (int pass = 0; pass < PASSES; pass++)
+  int passes = 3;
+  for (int pass = 0; pass < passes; pass++) {
+    float sum = 0;
     for (int v = 0; v < 4; v++) {
       prop_fn f = nlm_pick_prop_tab_variant(p->L, v);
       if (!f) continue;
@@ -781,12 +782,20 @@ int tune_variant(nlm_plan* p, const KParams& kp, int grid, int* best) {
       HIP_TRY(hipEventSynchronize(g_tune_ev[1]));
       float ms = 0;
       HIP_TRY(hipEventElapsedTime(&ms, g_tune_ev[0], g_tune_ev[1]));
+      sum += ms;
       if (pass > 0 && ms > worst[v]) worst[v] = ms;
     }
+    if (pass == 0) {
+      const int timed = sum > 0 ? (int)(60.f / sum) : 6;
+      passes = 1 + (timed < 2 ? 2 : timed > 6 ? 6 : timed);
+    }
+  }
   float best_ms = 1e30f;
   *best = 0;
+  // a later variant must beat the current choice by 1%, more than the
+  // launch-to-launch spread of one schedule, so that noise keeps variant 0
   for (int v = 0; v < 4; v++)
-    if (nlm_pick_prop_tab_variant(p->L, v) && worst[v] < best_ms) {
+    if (nlm_pick_prop_tab_variant(p->L, v) && worst[v] < 0.99f * best_ms) {
       best_ms = worst[v];
       *best = v;
     }
