@@ -181,6 +181,18 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
   const size_t perStep = (size_t)nWG * 2 * N;
   int chunk = (int)std::max<size_t>(1, ((size_t)1 << 31) / (perStep * sizeof(cplx)));
   if (chunk > nZ - 1) chunk = nZ - 1 > 0 ? nZ - 1 : 1;
+  // The reduction's (step, tile) blocks alone leave most CUs idle where one step
+  // has many workgroup partials (N = 4096: 2 steps x 16 tiles = 32 blocks reading
+  // 2 GiB per launch); below 256 such blocks a first level sums slices of at
+  // least 64 workgroups each (fixed order), ~1024 blocks per launch.
+  const int tilesR = N >= 256 ? N / 256 : (N + 63) / 64;
+  int per = nWG;
+  if (chunk * tilesR < 256) {
+    const int want = (1024 + chunk * tilesR - 1) / (chunk * tilesR);
+    const int S = std::max(1, std::min(want, nWG / 64));
+    per = (nWG + S - 1) / S;
+  }
+  const int slices = (nWG + per - 1) / per;
   const bool rows = g_pump_time || g_envelope;
   const int nEnv = std::min(nZP - 1 > 0 ? nZP - 1 : 1, 512);
   int e;
@@ -189,6 +201,7 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
       (e = p->stepTab.ensure((size_t)nZ * N * 2)) ||
       (e = p->gpumpPart.ensure(perStep * chunk)) || (e = p->gtmp.ensure(nF * 2 * N)))
     return e;
+  if (slices > 1 && (e = p->gslice.ensure((size_t)slices * chunk * 2 * N))) return e;
   if (rows && (e = p->gpump.ensure((size_t)nZP * N))) return e;
   if (g_envelope && ((e = p->gF0Part.ensure((size_t)nEnv * N)) || (e = p->gF0.ensure(3 * (size_t)N)))) return e;
   HIP_TRY(hipMemcpyAsync(p->gC.p, gC, total * sizeof(cplx), hipMemcpyHostToDevice, p->stream));
@@ -223,15 +236,25 @@ extern "C" int nlm_greens_adjoint(nlm_plan* p, const double* gC, const double* g
       ap.iLo = iHi - chunk > 0 ? iHi - chunk : 0;
       sweep(ap, nWG, p->stream);
       const int count = ap.iHi - ap.iLo;
-      if (N >= 256) {  // four points per lane: 1 KB per load instruction, 4 KB per wave and partial
-        const unsigned blocks = (unsigned)(count * (N / 256));
-        hipLaunchKernelGGL(k_adj_reduce<4>, dim3(blocks), dim3(64 * RED_WAVES), 0, p->stream, ap.part, nWG, chunk,
-                           ap.iLo, N, p->coef.p, p->pumpT[0].p, p->gcoefPart.p);
-      } else {
-        const unsigned blocks = (unsigned)(count * ((N + 63) / 64));
-        hipLaunchKernelGGL(k_adj_reduce<1>, dim3(blocks), dim3(64 * RED_WAVES), 0, p->stream, ap.part, nWG, chunk,
-                           ap.iLo, N, p->coef.p, p->pumpT[0].p, p->gcoefPart.p);
+      const unsigned blocks = (unsigned)(count * tilesR);
+      const cplx* sums = ap.part;
+      int nSums = nWG;
+      if (slices > 1) {
+        if (N >= 256)
+          hipLaunchKernelGGL(k_adj_slices<4>, dim3(blocks * slices), dim3(64 * RED_WAVES), 0, p->stream, ap.part,
+                             nWG, chunk, N, per, p->gslice.p);
+        else
+          hipLaunchKernelGGL(k_adj_slices<1>, dim3(blocks * slices), dim3(64 * RED_WAVES), 0, p->stream, ap.part,
+                             nWG, chunk, N, per, p->gslice.p);
+        sums = p->gslice.p;
+        nSums = slices;
       }
+      if (N >= 256)  // four points per lane: 1 KB per load instruction, 4 KB per wave and partial
+        hipLaunchKernelGGL(k_adj_reduce<4>, dim3(blocks), dim3(64 * RED_WAVES), 0, p->stream, sums, nSums, chunk,
+                           ap.iLo, N, p->coef.p, p->pumpT[0].p, p->gcoefPart.p);
+      else
+        hipLaunchKernelGGL(k_adj_reduce<1>, dim3(blocks), dim3(64 * RED_WAVES), 0, p->stream, sums, nSums, chunk,
+                           ap.iLo, N, p->coef.p, p->pumpT[0].p, p->gcoefPart.p);
     }
   }
   HIP_TRY(hipGetLastError());

@@ -367,6 +367,90 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
   }
 }
 
+constexpr int RED_WAVES = 8;
+constexpr int RED_ACC = 4;
+
+// Wave q of a block sums the partials of workgroups wLo + q, wLo + q + RED_WAVES,
+// ... < wHi (fixed order, RED_ACC independent accumulators so that many loads are
+// in flight per point) for step jj of the launch and the block's 64 TPL points
+// from k0, into red[0 / 1][q].
+template <int TPL>
+__device__ __forceinline__ void sum_partials(const cplx* __restrict__ part, int wLo, int wHi, int chunk, int jj,
+                                             int N, int k0, int q, int lane, bool valid,
+                                             cplx (&red)[2][RED_WAVES][64 * TPL]) {
+  cplx a4[TPL][RED_ACC], b4[TPL][RED_ACC];
+#pragma unroll
+  for (int u = 0; u < TPL; u++)
+#pragma unroll
+    for (int v = 0; v < RED_ACC; v++) a4[u][v] = b4[u][v] = mk(0, 0);
+  if (valid) {
+    const size_t stride = (size_t)RED_WAVES * chunk * 2 * N;
+    int w = wLo + q;
+    const cplx* p = part + (((size_t)w * chunk + jj) * 2) * N + k0 + lane;
+    for (; w + (RED_ACC - 1) * RED_WAVES < wHi; w += RED_ACC * RED_WAVES, p += RED_ACC * stride) {
+#pragma unroll
+      for (int v = 0; v < RED_ACC; v++)
+#pragma unroll
+        for (int u = 0; u < TPL; u++) {
+          a4[u][v] = a4[u][v] + load_stream(p + v * stride + 64 * u);
+          b4[u][v] = b4[u][v] + load_stream(p + v * stride + N + 64 * u);
+        }
+    }
+    for (int v = 0; w < wHi; w += RED_WAVES, p += stride, v++)
+#pragma unroll
+      for (int u = 0; u < TPL; u++) {
+        a4[u][v] = a4[u][v] + load_stream(p + 64 * u);
+        b4[u][v] = b4[u][v] + load_stream(p + N + 64 * u);
+      }
+  }
+#pragma unroll
+  for (int u = 0; u < TPL; u++) {
+#pragma unroll
+    for (int h = RED_ACC / 2; h >= 1; h /= 2) {
+#pragma unroll
+      for (int v = 0; v < h; v++) {
+        a4[u][v] = a4[u][v] + a4[u][v + h];
+        b4[u][v] = b4[u][v] + b4[u][v + h];
+      }
+    }
+    red[0][q][lane + 64 * u] = a4[u][0];
+    red[1][q][lane + 64 * u] = b4[u][0];
+  }
+}
+
+// First level of the reduction where one step's workgroups are too many for the
+// (step, tile) blocks alone to fill the device (N = 2048 and above): block
+// (jj, tile, s) sums the workgroups of slice s, [s per, (s + 1) per), in the
+// fixed order above and writes the slice total to out[s][jj][2][N], the layout
+// of the partials, so that k_adj_reduce then sums the slices (fixed order).
+template <int TPL>
+__global__ void __launch_bounds__(64 * RED_WAVES) k_adj_slices(const cplx* __restrict__ part, int nWG, int chunk,
+                                                               int N, int per, cplx* __restrict__ out) {
+  constexpr int TILE = 64 * TPL;
+  __shared__ cplx red[2][RED_WAVES][TILE];
+  const int tiles = (N + TILE - 1) / TILE, slices = (nWG + per - 1) / per;
+  const int s = blockIdx.x % slices, rest = blockIdx.x / slices;
+  const int jj = rest / tiles, k0 = (rest % tiles) * TILE;
+  const int q = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const bool valid = TPL > 1 || k0 + lane < N;
+  const int wLo = s * per, wHi = wLo + per < nWG ? wLo + per : nWG;
+  sum_partials<TPL>(part, wLo, wHi, chunk, jj, N, k0, q, lane, valid, red);
+  __syncthreads();
+  if (q != 0 || !valid) return;
+  cplx* o = out + (((size_t)s * chunk + jj) * 2) * N + k0 + lane;
+#pragma unroll
+  for (int u = 0; u < TPL; u++) {
+    cplx ga = red[0][0][lane + 64 * u], gb = red[1][0][lane + 64 * u];
+#pragma unroll
+    for (int r = 1; r < RED_WAVES; r++) {
+      ga = ga + red[0][r][lane + 64 * u];
+      gb = gb + red[1][r][lane + 64 * u];
+    }
+    o[64 * u] = ga;
+    o[N + 64 * u] = gb;
+  }
+}
+
 // Sum the partials of steps iLo+1 .. iLo+count and apply the chain rule from
 // (alpha, beta) to m_s = c_s P_s: R(x) = x + (k1 + 2 k2 + 2 k3 + k4) / 6 with
 // k_s = m_s conj(y_s) (Chi2PDC::DiffEq, solvers/Chi2PDC.cpp:27-47),
@@ -375,8 +459,6 @@ __global__ void __launch_bounds__(FA * FftShape<L>::T) k_adj_sweep(AParams ap) {
 // q, q + RED_WAVES, ... (fixed order), lane l taking the points k0 + l + 64 u
 // (u < TPL, so each load instruction of a wave reads 1 KB contiguous and a wave
 // streams 64 TPL points of each partial); then wave 0 adds the wave sums in order.
-constexpr int RED_WAVES = 8;
-constexpr int RED_ACC = 4;
 template <int TPL>
 __global__ void __launch_bounds__(64 * RED_WAVES) k_adj_reduce(const cplx* __restrict__ part, int nWG, int chunk,
                                                                int iLo, int N, const cplx* __restrict__ coef,
@@ -388,49 +470,7 @@ __global__ void __launch_bounds__(64 * RED_WAVES) k_adj_reduce(const cplx* __res
   const int q = threadIdx.x / 64, lane = threadIdx.x % 64;
   // TPL > 1 is launched only when TILE divides N
   const bool valid = TPL > 1 || k0 + lane < N;
-  {
-    // RED_ACC independent accumulators (workgroups w + RED_WAVES v of each group
-    // of RED_ACC RED_WAVES) keep that many loads in flight per point; the
-    // summation order is fixed
-    cplx a4[TPL][RED_ACC], b4[TPL][RED_ACC];
-#pragma unroll
-    for (int u = 0; u < TPL; u++)
-#pragma unroll
-      for (int v = 0; v < RED_ACC; v++) a4[u][v] = b4[u][v] = mk(0, 0);
-    if (valid) {
-      const size_t stride = (size_t)RED_WAVES * chunk * 2 * N;
-      const cplx* p = part + (((size_t)q * chunk + jj) * 2) * N + k0 + lane;
-      int w = q;
-      for (; w + (RED_ACC - 1) * RED_WAVES < nWG; w += RED_ACC * RED_WAVES, p += RED_ACC * stride) {
-#pragma unroll
-        for (int v = 0; v < RED_ACC; v++)
-#pragma unroll
-          for (int u = 0; u < TPL; u++) {
-            a4[u][v] = a4[u][v] + load_stream(p + v * stride + 64 * u);
-            b4[u][v] = b4[u][v] + load_stream(p + v * stride + N + 64 * u);
-          }
-      }
-      for (int v = 0; w < nWG; w += RED_WAVES, p += stride, v++)
-#pragma unroll
-        for (int u = 0; u < TPL; u++) {
-          a4[u][v] = a4[u][v] + load_stream(p + 64 * u);
-          b4[u][v] = b4[u][v] + load_stream(p + N + 64 * u);
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < TPL; u++) {
-#pragma unroll
-      for (int h = RED_ACC / 2; h >= 1; h /= 2) {
-#pragma unroll
-        for (int v = 0; v < h; v++) {
-          a4[u][v] = a4[u][v] + a4[u][v + h];
-          b4[u][v] = b4[u][v] + b4[u][v + h];
-        }
-      }
-      red[0][q][lane + 64 * u] = a4[u][0];
-      red[1][q][lane + 64 * u] = b4[u][0];
-    }
-  }
+  sum_partials<TPL>(part, 0, nWG, chunk, jj, N, k0, q, lane, valid, red);
   __syncthreads();
   if (q != 0 || !valid) return;
 #pragma unroll 1

@@ -490,7 +490,7 @@ extern "C" int nlm_plan_destroy(nlm_plan* p) {
                           &p->G, &p->C, &p->S, &p->in, &p->out, &p->env, &p->F0, &p->dsp, &p->relI};
   for (auto* b : bufs) b->release();
   DevBuf<cplx>* abufs[] = {&p->ckpt, &p->gG, &p->gC, &p->gS, &p->gcoefPart, &p->gcoef, &p->gpumpPart, &p->gpump,
-                           &p->gF0Part, &p->gF0, &p->gtmp, &p->scale, &p->stepTab};
+                           &p->gF0Part, &p->gF0, &p->gtmp, &p->scale, &p->stepTab, &p->gslice};
   for (auto* b : abufs) b->release();
   if (p->bigExec) (void)hipGraphExecDestroy(p->bigExec);
   p->bigState.release();

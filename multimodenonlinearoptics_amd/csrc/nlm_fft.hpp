@@ -118,6 +118,19 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Ordering of a transform's LDS exchanges.  WAVE: the field's lanes and the whole
+// workgroup are one wave, whose LDS operations the CU performs in the order the
+// wave issues them, so the exchange needs neither a barrier nor a wait for its
+// writes before its reads; a wavefront-scope fence (no instruction) keeps the
+// compiler from moving the accesses across it.  Otherwise a workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void exchange_sync() {
+  if constexpr (WAVE)
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront", "local");
+  else
+    lds_barrier();
+}
+
 // In-place R-point DFT of v[o], v[o+1], ..., v[o+R-1]; outputs in natural order.
 template <int R, bool INV>
 __device__ __forceinline__ void dft_small(cplx* v) {
@@ -398,12 +411,12 @@ __device__ __forceinline__ void fft_field2(cplx (&v)[8], cplx (&u)[8], cplx* __r
 }
 
 // One field with twiddles from tw(s, idx) (see fft_field).
-template <int L, bool INV, int s, class TW>
+template <int L, bool INV, int s, bool WAVE = false, class TW>
 __device__ __forceinline__ void fft_stages1(cplx (&v)[8], cplx* __restrict__ buf, int t, const TW& tw) {
   using F = FftShape<L>;
   constexpr int N = F::N, T = F::T, S = F::S;
   constexpr int R = F::radix(s), P = F::before(s), Mp = N / P, M = Mp / R, NB = 8 / R;
-  lds_barrier();
+  exchange_sync<WAVE>();
 #pragma unroll
   for (int j = 0; j < NB; j++) {
     const int beta = t + T * j;
@@ -424,7 +437,7 @@ __device__ __forceinline__ void fft_stages1(cplx (&v)[8], cplx* __restrict__ buf
       for (int m = 0; m < 8; m++) v[m] = o[m];
     }
   } else {
-    lds_barrier();
+    exchange_sync<WAVE>();
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       const int beta = t + T * j;
@@ -433,11 +446,11 @@ __device__ __forceinline__ void fft_stages1(cplx (&v)[8], cplx* __restrict__ buf
       for (int c = 0; c < R; c++)
         buf[lpad<L>((g + P * c) * M + b)] = c == 0 ? v[j * R] : twmul<INV>(v[j * R + c], tw(s, j * R + c));
     }
-    fft_stages1<L, INV, s + 1>(v, buf, t, tw);
+    fft_stages1<L, INV, s + 1, WAVE>(v, buf, t, tw);
   }
 }
 
-template <int L, bool INV, class TW>
+template <int L, bool INV, bool WAVE = false, class TW>
 __device__ __forceinline__ void fft_field1(cplx (&v)[8], cplx* __restrict__ buf, int t, const TW& tw) {
   using F = FftShape<L>;
   constexpr int T = F::T, S = F::S;
@@ -445,9 +458,9 @@ __device__ __forceinline__ void fft_field1(cplx (&v)[8], cplx* __restrict__ buf,
   if constexpr (S > 1) {
 #pragma unroll
     for (int c = 1; c < 8; c++) v[c] = twmul<INV>(v[c], tw(0, c));
-    lds_barrier();
+    exchange_sync<WAVE>();
 #pragma unroll
     for (int c = 0; c < 8; c++) buf[lpad<L>(c * T + t)] = v[c];
-    fft_stages1<L, INV, 1>(v, buf, t, tw);
+    fft_stages1<L, INV, 1, WAVE>(v, buf, t, tw);
   }
 }

@@ -63,7 +63,7 @@ struct nlm_plan_s {
   // Green's forward kept for the adjoint
   bool pumpLinear = false, hasStrength = false;
   double dzp = 0;
-  DevBuf<cplx> ckpt, gG, gC, gS, gcoefPart, gcoef, gpumpPart, gpump, gF0Part, gF0, gtmp;
+  DevBuf<cplx> ckpt, gG, gC, gS, gcoefPart, gcoef, gpumpPart, gpump, gF0Part, gF0, gtmp, gslice;
   DevBuf<cplx> scale;
   // staged transform for Chi3GNLSE with nFreqs > 8192 (nlm_big.hip)
   DevBuf<cplx> bigState, bigH;

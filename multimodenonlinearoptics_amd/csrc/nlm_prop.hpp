@@ -66,6 +66,12 @@ struct Launch {
   static constexpr int THREADS = FPB * T;
 };
 
+// Up to N = 512 every kernel that runs the steps below (k_prop, k_prop_tab,
+// k_prop_gnlse: Launch; k_prop_multi: one field of T lanes) is one wave per
+// workgroup, so their transforms exchange through LDS without barriers.
+template <int L>
+constexpr bool kWaveFft = FftShape<L>::T <= 64;
+
 // Row i of mode q of the z-resolved grid of `field` ([nFields][nS][nZ][N]).
 __device__ __forceinline__ cplx* grid_row(const KParams& kp, cplx* base, int field, int nS, int q, int i, int N) {
   return base + (((size_t)field * nS + q) * kp.nZ + i) * N;
@@ -151,7 +157,7 @@ __device__ __forceinline__ void prologue(cplx (&a)[NS][8], const KParams& kp, cp
   constexpr int N = FftShape<L>::N;
 #pragma unroll
   for (int q = 0; q < NS; q++) {
-    if (kp.inTime) fft_field1<L, false>(a[q], buf, t, tw);
+    if (kp.inTime) fft_field1<L, false, kWaveFft<L>>(a[q], buf, t, tw);
     mul_table<L>(a[q], kp.halfIn + (size_t)q * N, t);
     if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, NS, q, 0, N), a[q], t, active);
   }
@@ -169,13 +175,13 @@ __device__ __forceinline__ void epilogue(cplx (&a)[NS][8], const KParams& kp, cp
     mul_table<L>(a[q], kp.halfOut + (size_t)q * N, t);
     if (kp.kind == KIND_GRID) {
       store_row<L>(grid_row(kp, kp.gridF, field, NS, q, last, N), a[q], t, active);
-      fft_field1<L, true>(a[q], buf, t, tw);
+      fft_field1<L, true, kWaveFft<L>>(a[q], buf, t, tw);
       scale_inv<L>(a[q]);
       store_row<L>(grid_row(kp, kp.gridT, field, NS, q, last, N), a[q], t, active);
     } else {
       const int op = kp.outPos[q];
       if (kp.outTime) {
-        fft_field1<L, true>(a[q], buf, t, tw);
+        fft_field1<L, true, kWaveFft<L>>(a[q], buf, t, tw);
         scale_inv<L>(a[q]);
       }
       if (op >= 0) store_row<L>(kp.out + (size_t)field * kp.nOut * N + (size_t)op * N, a[q], t, active);
@@ -209,7 +215,7 @@ struct InlinePump {
       e1[m] = mk(cs, sn);
       ph[m] = mk(1.0, 0.0);
     }
-    fft_field1<L, false>(F0, buf, t, tw);
+    fft_field1<L, false, kWaveFft<L>>(F0, buf, t, tw);
     row = 0;
   }
   template <class TW>
@@ -229,7 +235,7 @@ struct InlinePump {
       }
       out[m] = F0[m] * ph[m];
     }
-    fft_field1<L, true>(out, buf, t, tw);
+    fft_field1<L, true, kWaveFft<L>>(out, buf, t, tw);
     const double s = (kp.strength ? kp.strength[row] : 1.0) * (1.0 / FftShape<L>::N);
 #pragma unroll
     for (int m = 0; m < 8; m++) out[m] = out[m] * s;
@@ -281,7 +287,7 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
   if (kp.nZ > 1) {
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-      fft_field1<L, true>(a[q], buf, t, tw);
+      fft_field1<L, true, kWaveFft<L>>(a[q], buf, t, tw);
       scale_inv<L>(a[q]);
       if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, NS, q, 0, N), a[q], t, active);
       if (kp.ckpt) store_row<L>(kp.ckpt + ((size_t)field * kp.nSeg * NS + q) * N, a[q], t, active);
@@ -379,13 +385,13 @@ __device__ __forceinline__ void prop_field(const KParams& kp, int field, bool ac
     // --- dispersion step (src/_NonlinearMedium.hpp:212-215)
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-      fft_field1<L, false>(a[q], buf, t, tw);
+      fft_field1<L, false, kWaveFft<L>>(a[q], buf, t, tw);
 #pragma unroll
       for (int m = 0; m < 8; m++)
         a[q][m] = a[q][m] * (LEAN ? kp.dstep[(size_t)q * N + t + T * m] : ds[q][LEAN ? 0 : m]);
       if (i < kp.nZ - 1) {
         if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, NS, q, i, N), a[q], t, active);
-        fft_field1<L, true>(a[q], buf, t, tw);
+        fft_field1<L, true, kWaveFft<L>>(a[q], buf, t, tw);
         scale_inv<L>(a[q]);
         if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, NS, q, i, N), a[q], t, active);
         if (kp.ckpt && i % kp.ckptEvery == 0)
@@ -424,17 +430,17 @@ __device__ __forceinline__ void gnlse_stage(const cplx (&y)[8], double r, cplx (
                                             const cplx (&flt)[8], cplx* buf, int t, const TW& tw) {
 #pragma unroll
   for (int m = 0; m < 8; m++) k[m] = mk(r * abs2(y[m]), 0.0);
-  fft_field1<L, false>(k, buf, t, tw);
+  fft_field1<L, false, kWaveFft<L>>(k, buf, t, tw);
 #pragma unroll
   for (int m = 0; m < 8; m++) k[m] = k[m] * ram[m];
-  fft_field1<L, true>(k, buf, t, tw);
+  fft_field1<L, true, kWaveFft<L>>(k, buf, t, tw);
   scale_inv<L>(k);
 #pragma unroll
   for (int m = 0; m < 8; m++) k[m] = k[m] * y[m];
-  fft_field1<L, false>(k, buf, t, tw);
+  fft_field1<L, false, kWaveFft<L>>(k, buf, t, tw);
 #pragma unroll
   for (int m = 0; m < 8; m++) k[m] = k[m] * flt[m];
-  fft_field1<L, true>(k, buf, t, tw);
+  fft_field1<L, true, kWaveFft<L>>(k, buf, t, tw);
   scale_inv<L>(k);
 }
 
@@ -458,7 +464,7 @@ __device__ __forceinline__ void gnlse_field(const KParams& kp, int field, bool a
   load_inputs<L, 1>(a, kp, field, t, active);
   prologue<L, 1>(a, kp, buf, t, tw, field, active);
   if (kp.nZ > 1) {
-    fft_field1<L, true>(a[0], buf, t, tw);
+    fft_field1<L, true, kWaveFft<L>>(a[0], buf, t, tw);
     scale_inv<L>(a[0]);
     if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, 1, 0, 0, N), a[0], t, active);
   }
@@ -490,12 +496,12 @@ __device__ __forceinline__ void gnlse_field(const KParams& kp, int field, bool a
 #pragma unroll
     for (int m = 0; m < 8; m++) a[0][m] = a[0][m] + (acc[m] + k[m]) * (1. / 6.);
 
-    fft_field1<L, false>(a[0], buf, t, tw);
+    fft_field1<L, false, kWaveFft<L>>(a[0], buf, t, tw);
 #pragma unroll
     for (int m = 0; m < 8; m++) a[0][m] = a[0][m] * ds[m];
     if (i < kp.nZ - 1) {
       if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, 1, 0, i, N), a[0], t, active);
-      fft_field1<L, true>(a[0], buf, t, tw);
+      fft_field1<L, true, kWaveFft<L>>(a[0], buf, t, tw);
       scale_inv<L>(a[0]);
       if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridT, field, 1, 0, i, N), a[0], t, active);
     }

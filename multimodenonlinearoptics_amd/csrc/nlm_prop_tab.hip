@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
   prologue<L, 1>(a, kp, buf, t, tw, field, active);
   cplx al[8], be[8];
   if (kp.nZ > 1) {
-    fft_field1<L, true>(a[0], buf, t, tw);  // unscaled: the table holds the 1/N
+    fft_field1<L, true, kWaveFft<L>>(a[0], buf, t, tw);  // unscaled: the table holds the 1/N
     if (kp.kind == KIND_GRID) store_scaled<L>(grid_row(kp, kp.gridT, field, 1, 0, 0, N), a[0], t, active);
     if (kp.ckpt) store_scaled<L>(kp.ckpt + (size_t)field * kp.nSeg * N, a[0], t, active);
     load_prop_row<L>(al, be, kp.tab + (size_t)N * 2, t);
@@ -127,13 +127,13 @@ __global__ void __launch_bounds__(Launch<L>::THREADS) k_prop_tab(KParams kp) {
     if (!LATE && i + 1 < kp.nZ) load_prop_row<L>(al, be, kp.tab + (size_t)(i + 1) * N * 2, t);
 
     // --- dispersion step (src/_NonlinearMedium.hpp:212-215)
-    fft_field1<L, false>(a[0], buf, t, tw);
+    fft_field1<L, false, kWaveFft<L>>(a[0], buf, t, tw);
 #pragma unroll
     for (int m = 0; m < 8; m++) a[0][m] = a[0][m] * (DS_MEM ? kp.dstep[t + T * m] : ds[DS_MEM ? 0 : m]);
     if (LATE && i + 1 < kp.nZ) load_prop_row<L>(al, be, kp.tab + (size_t)(i + 1) * N * 2, t);
     if (i < kp.nZ - 1) {
       if (kp.kind == KIND_GRID) store_row<L>(grid_row(kp, kp.gridF, field, 1, 0, i, N), a[0], t, active);
-      fft_field1<L, true>(a[0], buf, t, tw);
+      fft_field1<L, true, kWaveFft<L>>(a[0], buf, t, tw);
       if (kp.kind == KIND_GRID) store_scaled<L>(grid_row(kp, kp.gridT, field, 1, 0, i, N), a[0], t, active);
       if (kp.ckpt && i % kp.ckptEvery == 0)
         store_scaled<L>(kp.ckpt + ((size_t)field * kp.nSeg + i / kp.ckptEvery) * N, a[0], t, active);
