@@ -58,8 +58,8 @@ def flops_per_field_step(n):
     return 10 * n * math.log2(n) + 80 * n
 
 
-def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4, ckpt_every=0):
-    """Differentiable Chi2PDC (N=1024, nZ=2000): loss ||S S^T - 1.5 I||_F^2 of the
+def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4, ckpt_every=0, n=1024, nz=2000):
+    """Differentiable Chi2PDC (N=1024, nZ=2000 by default): loss ||S S^T - 1.5 I||_F^2 of the
     Green's function S, gradient w.r.t. the pump spectral phase (1024 reals) and the
     per-step poling (2000 reals) by the adjoint kernel with ~sqrt(nZ) checkpoints.
 
@@ -73,10 +73,11 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4, ckpt_ev
                    and 2e-3 (truncation O(eps^4), rounding ~1e-13 / eps);
       forward mode torch forward-mode AD of the torch restatement (tests/pdc_torch.py,
                    pinned to the oracle) on fwd_mode_dirs directions, run in a
-                   process of its own."""
+                   process of its own.
+    n_dirs = 0 times the forward and the backward only (other sizes)."""
     import numpy as np
     from cases import config4, config4_adjoint, config4_directions, config4_fd, config4_parts
-    m, spec0, phi, pol = config4(nlm)
+    m, spec0, phi, pol = config4(nlm, n, nz)
     N, nZ = spec0.size, pol.size
     m.enableTiming(True)
     m.greensOnDevice()
@@ -100,6 +101,18 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4, ckpt_ev
         t_bwd.append(time.perf_counter() - t0)
         tm = m.lastTiming()
         k_bwd.append(tm[1] + tm[2])
+    timing = {
+        "workload": "Chi2PDC N=%d nZ=%d, loss ||S S^T - 1.5 I||_F^2, grad w.r.t. pump spectral phase "
+                    "(%d) and per-step poling (%d)" % (N, nZ, N, nZ),
+        "forward_s": min(t_fwd), "forward_ckpt_s": min(t_ck), "backward_s": min(t_bwd),
+        "forward_kernel_ms": min(k_fwd), "backward_kernel_ms": min(k_bwd),
+        # device work of the backward (sweep, reductions, chain rule) over that of the
+        # forward solve; the wall-clock ratio also holds the upload of dL/dC, dL/dS
+        "grad_vs_forward": min(k_bwd) / min(k_fwd),
+        "grad_vs_forward_wall": min(t_bwd) / min(t_fwd),
+    }
+    if n_dirs <= 0:
+        return timing
     L0, g_phi, g_pol = config4_adjoint(m, spec0, phi, pol, ckpt_every=ckpt_every)
     dirs = config4_directions(n_dirs, N, nZ)
     ads = [float(np.dot(g_phi, dp) + np.dot(g_pol, dq)) for dp, dq in dirs]
@@ -117,16 +130,9 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4, ckpt_ev
 
     fwd = forward_mode_reference(config4_parts(N, nZ)[0], spec0, phi, pol, dirs[:fwd_mode_dirs]) \
         if fwd_mode_dirs > 0 else None
-    out = {
-        "workload": "Chi2PDC N=1024 nZ=2000, loss ||S S^T - 1.5 I||_F^2, grad w.r.t. pump spectral phase "
-                    "(1024) and per-step poling (2000)",
+    out = dict(timing)
+    out.update({
         "loss": L0,
-        "forward_s": min(t_fwd), "forward_ckpt_s": min(t_ck), "backward_s": min(t_bwd),
-        "forward_kernel_ms": min(k_fwd), "backward_kernel_ms": min(k_bwd),
-        # device work of the backward (sweep, reductions, chain rule) over that of the
-        # forward solve; the wall-clock ratio also holds the upload of dL/dC, dL/dS
-        "grad_vs_forward": min(k_bwd) / min(k_fwd),
-        "grad_vs_forward_wall": min(t_bwd) / min(t_fwd),
         "directions": n_dirs,
         "fd_rich_max_rel_err": max(rel(a, r) for a, r in zip(ads, rich)),
         "fd_rich_steps": [4e-3, 2e-3],
@@ -135,7 +141,7 @@ def grad_benchmark(nlm, reps=3, n_dirs=32, fd_eps=1e-6, fwd_mode_dirs=4, ckpt_ev
         "fd_eps_noise_floor": max(rel(f, r) for f, r in zip(raw, rich)),
         "pairs": [[a, r, f] for a, r, f in zip(ads[:6], rich[:6], raw[:6])],
         "grad_norm": float(np.sqrt(np.sum(g_phi ** 2) + np.sum(g_pol ** 2))),
-    }
+    })
     if fwd is not None:
         out["forward_mode_directions"] = len(fwd)
         out["forward_mode_max_rel_err"] = max(rel(a, d) for a, (_, d) in zip(ads, fwd))
@@ -474,7 +480,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    # the GPU's clocks rise over the first ~20 ms of work (2.36 -> 2.03 ms per
+    # headline launch over its first ten launches): 20 untimed solves, ~45 ms
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint (gradient) measurement")
@@ -492,6 +500,8 @@ def main():
                     help="threads for the CPU baseline (default: every core this process may use, at most 16)")
     ap.add_argument("--grad-dirs", type=int, default=32,
                     help="seeded directions of the config-4 finite-difference check")
+    ap.add_argument("--grad-n", type=int, default=1024, help="nFreqs of the config-4 medium (--only grad)")
+    ap.add_argument("--grad-nz", type=int, default=2000, help="nZSteps of the config-4 medium (--only grad)")
     ap.add_argument("--ckpt-every", type=int, default=0,
                     help="checkpoint interval of the config-4 forward (0: the engine's choice)")
     ap.add_argument("--fwd-mode-dirs", type=int, default=4,
@@ -546,7 +556,7 @@ def main():
             out = compose_benchmark()
         else:
             out = grad_benchmark(nlm, n_dirs=args.grad_dirs, fwd_mode_dirs=args.fwd_mode_dirs,
-                                 ckpt_every=args.ckpt_every)
+                                 ckpt_every=args.ckpt_every, n=args.grad_n, nz=args.grad_nz)
         print(json.dumps(out))
         return
     medium = config2(nlm, N_FREQS, NZ)
@@ -571,7 +581,8 @@ def main():
     value = field_steps / (ms_per_step * 1e-3)
 
     prop_avg = sum(prop_ms) / len(prop_ms)
-    # the FFT schedule the autotuner chose for this launch shape (kernel k_prop_tab<9 + 32 v>)
+    # the FFT schedule the autotuner chose for this launch shape (kernel k_prop_tab<9 + 32 v>;
+    # -1: not tuned, the default schedule 0)
     from multimodenonlinearoptics_amd import capi
     sched = int(capi.lib().nlm_fft_schedule(N_FREQS, 0, 2 * N_FREQS))
     # end to end: computeGreensFunction with C and S copied to host numpy arrays

@@ -735,8 +735,10 @@ int launch_prop_kernel(nlm_plan* p, const KParams& kp0) {
   return launch_prop_kernel_tab(p, kp0, tab, true);
 }
 
-// FFT schedule autotuner of the step-table propagation (nFreqs 512..8192 with at
-// least 128 fields): the first launch of a (device, nFreqs, kind, field-count
+// FFT schedule autotuner of the step-table propagation (nFreqs 1024..8192 with at
+// least 128 fields; at 512 schedule 0 was never slower once the clocks were up,
+// the other padding now and then 5-10 % slower from one launch to the next, which
+// a few timed launches do not reliably catch): the first launch of a (device, nFreqs, kind, field-count
 // bucket) runs every schedule variant of the kernel on the real inputs (the
 // launch is idempotent: same inputs, same outputs overwritten), keeps the fastest
 // by HIP events, and runs it once more so that the call's outputs are that
@@ -763,11 +765,10 @@ int tune_variant(nlm_plan* p, const KParams& kp, int grid, int* best) {
     HIP_TRY(hipEventCreate(&g_tune_ev[0]));
     HIP_TRY(hipEventCreate(&g_tune_ev[1]));
   }
-  // the first pass warms the clocks and the code objects; a variant scores the
-  // slowest of its timed launches, so that one whose time varies from launch to
-  // launch is not chosen on fast samples (measured at N = 512: the padding
-  // variant 1 took 2.03 or 2.25 ms from one launch to the next, variant 0
-  // 2.02-2.07 ms).  As many timed passes as fit ~60 ms, 2 to 6.
+  // the first pass warms the code objects (and starts the clocks, which keep
+  // rising over the next few launches); a variant scores the slowest of its timed
+  // launches, so that one whose time varies from launch to launch is not chosen
+  // on fast samples.  As many timed passes as fit ~60 ms, 2 to 6.
   float worst[4] = {0, 0, 0, 0};
   int passes = 3;
   for (int pass = 0; pass < passes; pass++) {
@@ -820,7 +821,7 @@ int launch_prop_kernel_tab(nlm_plan* p, const KParams& kp0, cplx* tab, bool may_
     const int forced = env_int("NLM_FFT_SCHEDULE", -1);
     if (forced >= 0) {
       if (prop_fn f = nlm_pick_prop_tab_variant(p->L, forced)) fn = f;
-    } else if (p->L >= 9 && kp.nFields >= 128 && env_int("NLM_AUTOTUNE", 1) != 0) {
+    } else if (p->L >= 10 && kp.nFields >= 128 && env_int("NLM_AUTOTUNE", 1) != 0) {
       std::lock_guard<std::mutex> lock(g_tune_mu);
       const long long key = tune_key(p, kp);
       auto it = g_tuned.find(key);

@@ -37,14 +37,18 @@ def test_schedule_variants_match_oracle(nlm, oracle, monkeypatch, N, nZ, variant
 def test_autotuner_records_a_choice(nlm):
     from multimodenonlinearoptics_amd import capi
     L = capi.lib()
-    m = nlm.Chi2PDC(**_kw(512, 60))
+    m = nlm.Chi2PDC(**_kw(1024, 60))
     C0, S0 = m.computeGreensFunction()
-    v = L.nlm_fft_schedule(512, 0, 1024)
+    v = L.nlm_fft_schedule(1024, 0, 2048)
     assert 0 <= v <= 3
     # later solves reuse the choice and reproduce the first one's outputs exactly
     C1, S1 = m.computeGreensFunction()
     assert np.array_equal(C0, C1) and np.array_equal(S0, S1)
-    assert L.nlm_fft_schedule(512, 0, 1024) == v
+    assert L.nlm_fft_schedule(1024, 0, 2048) == v
+    # N = 512 is not tuned: it keeps schedule 0
+    m512 = nlm.Chi2PDC(**_kw(512, 60))
+    m512.computeGreensFunction()
+    assert L.nlm_fft_schedule(512, 0, 1024) == -1
 
 
 
