@@ -97,6 +97,13 @@ def build(force=False, jobs=None, verbose=False):
         if force or _stale(o, deps):
             steps.append([HIPCC] + HIP_FLAGS + ["-DNLM_TAB_VARIANT=%d" % v, "-c",
                                                 os.path.join(CSRC, "nlm_prop_tab.hip"), "-o", o])
+    # the single-wave sizes of the step-table kernel, with the memory-clause scheduler
+    o = os.path.join(OBJ, "nlm_prop_tab_small.o")
+    objs.append(o)
+    deps = _deps("nlm_prop_tab.hip", *dev_hdrs)
+    if force or _stale(o, deps):
+        steps.append([HIPCC] + HIP_FLAGS + ["-mllvm", "-amdgpu-sched-strategy=max-memory-clause", "-DNLM_TAB_SMALL",
+                                            "-c", os.path.join(CSRC, "nlm_prop_tab.hip"), "-o", o])
     for m in PROP_MODELS:
         o = os.path.join(OBJ, "nlm_prop_%d.o" % m)
         objs.append(o)

@@ -155,12 +155,14 @@ static void launch_prop_tab(const KParams& kp, int grid, hipStream_t s) {
 #define NLM_CAT2(a, b) a##b
 #define NLM_CAT(a, b) NLM_CAT2(a, b)
 
-prop_fn NLM_CAT(nlm_pick_prop_tab_var, NLM_TAB_VARIANT)(int L) {
-  constexpr int V = NLM_TAB_VARIANT, C = 32 * V;
-  // bit 1 (small radix last) only differs from the default when log2 N is not a multiple of 3
-  if (V != 0 && (L < 9 || L > 13 || ((V & 2) && L % 3 == 0))) return nullptr;
+#ifdef NLM_TAB_SMALL
+// The single-wave sizes (nFreqs 8..512, schedule 0) are compiled in a translation
+// unit of their own, with LLVM's memory-clause machine scheduler: at one wave per
+// SIMD nothing hides the row loads and LDS exchanges, and its order was 4 % faster
+// at N = 512 (1.95 against 2.03 ms per Green's launch, same box), while from
+// N = 1024 (two or more waves per SIMD) it was 1-5 % slower.
+prop_fn nlm_pick_prop_tab_small(int L) {
   switch (L) {
-#if NLM_TAB_VARIANT == 0
     case 3: return &launch_prop_tab<3>;
     case 4: return &launch_prop_tab<4>;
     case 5: return &launch_prop_tab<5>;
@@ -168,6 +170,25 @@ prop_fn NLM_CAT(nlm_pick_prop_tab_var, NLM_TAB_VARIANT)(int L) {
     case 7: return &launch_prop_tab<7>;
     case 8: return &launch_prop_tab<8>;
     case 9: return &launch_prop_tab<9>;
+  }
+  return nullptr;
+}
+#else
+prop_fn nlm_pick_prop_tab_small(int L);
+
+prop_fn NLM_CAT(nlm_pick_prop_tab_var, NLM_TAB_VARIANT)(int L) {
+  constexpr int V = NLM_TAB_VARIANT, C = 32 * V;
+  // bit 1 (small radix last) only differs from the default when log2 N is not a multiple of 3
+  if (V != 0 && (L < 9 || L > 13 || ((V & 2) && L % 3 == 0))) return nullptr;
+  switch (L) {
+#if NLM_TAB_VARIANT == 0
+    case 3:
+    case 4:
+    case 5:
+    case 6:
+    case 7:
+    case 8:
+    case 9: return nlm_pick_prop_tab_small(L);
     case 12: return &launch_prop_tab<12>;
 #elif (NLM_TAB_VARIANT & 2) == 0
     case 9: return &launch_prop_tab<9 + C>;
@@ -214,3 +235,4 @@ bool nlm_launch_step_table(int model, const cplx* coef, const cplx* cconst, cons
   return true;
 }
 #endif  // NLM_TAB_VARIANT == 0
+#endif  // NLM_TAB_SMALL
