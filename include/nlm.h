@@ -298,7 +298,8 @@ int nlm_fft(int n, int count, const double* in, double* out, int inverse);
 /* Extension: the FFT schedule variant (0..3, see csrc/nlm_fft.hpp FftShape) the
  * autotuner chose on the current device for step-table propagations of n_freqs
  * points, kind (0 Green's function, 1 batch, 2 z-resolved grid) and a field count
- * of n_fields (rounded up to a power of two); -1 if none has been tuned. */
+ * of n_fields (rounded up to a power of two); -1 if none has been tuned (always
+ * for n_freqs below 1024, which run schedule 0). */
 int nlm_fft_schedule(int n_freqs, int kind, int n_fields);
 
 /* Synchronise the plan's stream. */
