@@ -488,6 +488,10 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint (gradient) measurement")
     ap.add_argument("--no-hetero", action="store_true", help="skip the heterogeneous-batch measurement")
     ap.add_argument("--hetero-count", type=int, default=2000)
+    ap.add_argument("--hetero-plain", type=int, default=40,
+                    help="instances of the batch run one by one for the plain comparison (--only hetero)")
+    ap.add_argument("--hetero-cpu", type=int, default=8,
+                    help="costliest and random instances checked against the oracle (--only hetero)")
     ap.add_argument("--no-gnlse", action="store_true", help="skip the nFreqs=16384 Chi3GNLSE measurement")
     ap.add_argument("--no-gnlse-grad", action="store_true", help="skip the Chi3GNLSE input-gradient measurement")
     ap.add_argument("--no-compose", action="store_true", help="skip the Green's composition measurement")
@@ -545,7 +549,8 @@ def main():
     nlm = pkg.nonlinearmedium
     if args.only:
         if args.only == "hetero":
-            out = hetero_benchmark(nlm, count=args.hetero_count, cpu_cores=cores)
+            out = hetero_benchmark(nlm, count=args.hetero_count, cpu_cores=cores, plain_sample=args.hetero_plain,
+                                   cpu_sample=args.hetero_cpu)
         elif args.only == "config1":
             out = config1_benchmark(nlm, O)
         elif args.only == "gnlse":
